@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Training throughput of the gfx950 Gaussian-splatting path on one MI355X.
+
+Workload: 3M Gaussians (seeded 64-blob mixture, scales from the GPU 3-NN initialisation),
+1920x1080, SH degree 3, 32 cameras on a radius-5 sphere with targets rendered from the ground
+truth and resident in HBM. One step is one full training iteration of the reference schedule
+(render, L1+SSIM gradient, backward, Adam, statistics, density control at its 100-iteration
+cadence) starting at iteration 3000. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import pathlib
+import platform
+import sys
+import time
+
+import numpy as np
+
+REPO = pathlib.Path(__file__).resolve().parent
+METRIC = "iters/s + fwd/bwd ms @3M Gaussians 1080p; learned vs rule densify at equal PSNR"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    if "gs_native_amd" in sys.modules:
+        return sys.modules["gs_native_amd"]
+    package_dir = REPO / "gs-native_amd"
+    spec = importlib.util.spec_from_file_location("gs_native_amd", package_dir / "__init__.py",
+                                                  submodule_search_locations=[str(package_dir)])
+    module = importlib.util.module_from_spec(spec)
+    sys.modules["gs_native_amd"] = module
+    spec.loader.exec_module(module)
+    return module
+
+
+def log(message: str) -> None:
+    print(f"[bench] {message}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n: int, v: float, r: float, p: int, t: int, k: int = 16) -> dict:
+    """Compulsory HBM bytes per launch of each stage (one iteration)."""
+    return {
+        "project": 16 * n + v * (28 + 12 * k + 56),
+        "scan": 8 * n,
+        "depth_sort": 8 * n,
+        "emit": 24 * v + 12 * r,
+        "tile_sort": 24 * r,
+        "ranges": 8 * r + 8 * (t + 1),
+        "raster_forward": 40 * r + 12 * p + 8 * (t + 1),
+        "image_gradients": 27 * p,
+        "raster_backward": 40 * r + 24 * p + 44 * v,
+        "gaussian_gradients": 32 * n + 500 * v + 1624 * n,
+    }
+
+
+def build_workload(gsn, context, gaussians, width, height, cameras, seed=0):
+    scenes_spec = importlib.util.spec_from_file_location("gs_native_amd.scenes", REPO / "gs-native_amd" / "scenes.py")
+    scenes = importlib.util.module_from_spec(scenes_spec)
+    scenes_spec.loader.exec_module(scenes)
+    truth = scenes.blob_scene(gaussians, seed=seed, sh_degree=3)
+    _, mean_dist2 = gsn.knn3(context, truth[0:3].T.copy())
+    truth[3:6] = scenes.scales_from_mean_dist2(mean_dist2)[None, :]
+    params = scenes.perturb(truth, seed + 1, 0.01)
+    focal = scenes.fov_focal(width, 60.0)
+    poses = scenes.sphere_cameras(cameras, 5.0, seed + 2)
+    cams = [gsn.Camera.make(width, height, focal, focal, 0.5 * (width - 1), 0.5 * (height - 1), R, t) for R, t in poses]
+    centres = np.stack([-R.T @ t for R, t in poses])
+    extent = 1.1 * float(np.linalg.norm(centres - centres.mean(0), axis=1).max())
+    device_truth = gsn.DeviceBuffer.from_numpy(truth)
+    image = gsn.DeviceBuffer((height, width, 3), np.float32)
+    targets, visible, records = [], [], []
+    projection = None
+    for cam in cams:
+        projection = gsn.render(context, device_truth, gaussians, cam, (0.0, 0.0, 0.0), image, 3, projection)
+        target = np.clip(np.rint(image.numpy() * 255.0), 0, 255).astype(np.uint8)
+        targets.append(gsn.DeviceBuffer.from_numpy(target))
+    del device_truth
+    device_params = gsn.DeviceBuffer.from_numpy(params)
+    for cam in cams:
+        projection = gsn.render(context, device_params, gaussians, cam, (0.0, 0.0, 0.0), image, 3, projection)
+        counts = np.diff(projection.record_offsets.numpy())
+        visible.append(int((counts > 0).sum()))
+        records.append(int(projection.record_count))
+    del device_params
+    return params, cams, targets, extent, visible, records
+
+
+def cpu_baseline(seconds: float) -> dict:
+    """The oracle (C, one thread) on one full iteration of the 10k-Gaussian 256x256 SH0 case:
+    projection, render, L1+SSIM gradient, replay backward, chain rule and Adam."""
+    sys.path.insert(0, str(REPO))
+    import oracle
+    scenes_spec = importlib.util.spec_from_file_location("gsn_scenes_cpu", REPO / "gs-native_amd" / "scenes.py")
+    scenes = importlib.util.module_from_spec(scenes_spec)
+    scenes_spec.loader.exec_module(scenes)
+    params = scenes.config0_params(seed=0, count=10_000)
+    camera = oracle.Camera()
+    f = 128.0 / math.tan(math.radians(30.0))
+    camera.fx = camera.fy = f
+    camera.cx = camera.cy = 127.5
+    for k in range(9):
+        camera.R_c_w[k] = 1.0 if k % 4 == 0 else 0.0
+    camera.t_c_w[2] = 3.0
+    camera.near_depth, camera.far_depth = 0.2, float(np.finfo(np.float32).max)
+    camera.width = camera.height = 256
+    camera.tile_count_x = camera.tile_count_y = 16
+    truth = scenes.perturb(params, 1, 0.01)
+    target = np.clip(np.rint(oracle.render_reference(truth, camera, (0, 0, 0), 0) * 255.0), 0, 255).astype(np.uint8)
+    first = np.zeros_like(params)
+    second = np.zeros_like(params)
+
+    def one_iteration():
+        result = oracle.gradients_f64(params, camera, 0, (0.0, 0.0, 0.0), target)
+        oracle.adam_f32(result["gradients"].astype(np.float32), params, first, second, 1.6e-4, 10.0, 1000.0)
+
+    one_iteration()  # warm-up
+    times = []
+    start = time.perf_counter()
+    while len(times) < 5 or (time.perf_counter() - start < seconds and len(times) < 50):
+        t0 = time.perf_counter()
+        one_iteration()
+        times.append(time.perf_counter() - t0)
+    median = float(np.median(times))
+    return {"value": 1.0 / median, "unit": "iters/s", "cores": 1, "kind": "port",
+            "sample": f"oracle chain (FP64 render/backward/map + FP32 Adam), 10k Gaussians, 256x256, SH0, "
+                      f"median of {len(times)} iterations after 1 warm-up, single thread on {platform.processor() or platform.machine()}"}
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=100)
+    parser.add_argument("--warmup", type=int, default=5)
+    parser.add_argument("--gaussians", type=int, default=3_000_000)
+    parser.add_argument("--width", type=int, default=1920)
+    parser.add_argument("--height", type=int, default=1080)
+    parser.add_argument("--cameras", type=int, default=32)
+    parser.add_argument("--start-iteration", type=int, default=3000)
+    parser.add_argument("--cpu-seconds", type=float, default=15.0)
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build runs on a single "
+              f"MI355X; see DESIGN.md for how the path would shard).", file=sys.stderr)
+        return 2
+    gsn = load_package()
+    trainer_spec = importlib.util.spec_from_file_location("gs_native_amd.trainer", REPO / "gs-native_amd" / "trainer.py")
+    trainer_module = importlib.util.module_from_spec(trainer_spec)
+    trainer_spec.loader.exec_module(trainer_module)
+    context = gsn.Context()
+    log(f"device {gsn.device_name()}")
+    log(f"building workload: {args.gaussians} Gaussians, {args.width}x{args.height}, {args.cameras} cameras")
+    params, cameras, targets, extent, visible, records = build_workload(gsn, context, args.gaussians, args.width,
+                                                                         args.height, args.cameras)
+    log(f"workload ready: mean visible {np.mean(visible):.0f}, mean records {np.mean(records):.0f}")
+    trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=args.start_iteration)
+    for _ in range(args.warmup):
+        trainer.iteration()
+    context.synchronize()
+    log(f"warm-up done ({args.warmup} iterations), gaussians {trainer.count}")
+    context.set_timing(True)
+    start_count = trainer.count
+    record_total = 0
+    visible_total = 0
+    gaussian_total = 0
+    begin = time.perf_counter()
+    for step in range(args.steps):
+        gaussian_total += trainer.count
+        record_total += trainer.iteration()
+        visible_total += visible[trainer.last_camera]
+        if (step + 1) % 25 == 0:
+            log(f"step {step + 1}/{args.steps}")
+    context.synchronize()
+    elapsed = time.perf_counter() - begin
+    stages = context.stage_times()
+    context.set_timing(False)
+    ms_per_step = 1000.0 * elapsed / args.steps
+    n_mean = gaussian_total / args.steps
+    r_mean = record_total / args.steps
+    v_mean = visible_total / args.steps
+    p = args.width * args.height
+    t = ((args.width + 15) // 16) * ((args.height + 15) // 16)
+    bytes_per_launch = algorithmic_bytes(n_mean, v_mean, r_mean, p, t)
+    per_stage = {name: {"ms_per_iteration": ms / args.steps, "launches": launches}
+                 for name, (ms, launches) in stages.items() if launches}
+    candidates = {name: stages[name][0] for name in bytes_per_launch if stages[name][1]}
+    dominant = max(candidates, key=candidates.get)
+    launches = stages[dominant][1]
+    avg_ms = stages[dominant][0] / launches
+    achieved = bytes_per_launch[dominant] / (avg_ms * 1e-3) / 1e9
+    for name, info in per_stage.items():
+        if name in bytes_per_launch and info["launches"]:
+            per_launch_ms = stages[name][0] / info["launches"]
+            info["hbm_gbs_algorithmic"] = bytes_per_launch[name] / (per_launch_ms * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.gaussians // 1_000_000}M-Gaussian 64-blob scene, {args.width}x{args.height}, SH degree 3, "
+                        f"{args.cameras} cameras, training iterations {args.start_iteration + args.warmup + 1}-"
+                        f"{args.start_iteration + args.warmup + args.steps} of the reference schedule",
+            "gaussians_start": start_count,
+            "gaussians_end": trainer.count,
+            "mean_records": r_mean,
+            "mean_visible": v_mean,
+            "density_steps": [h for h in trainer.history],
+        },
+        "stages": per_stage,
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": avg_ms,
+            "algorithmic_bytes_per_launch": bytes_per_launch[dominant],
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,406 @@
+"""Python binding of libgsn_hip.so, the gfx950 Gaussian-splatting training path.
+
+The library is loaded with ctypes; device memory is handled through the library's own
+plumbing (gsn_device_malloc / gsn_memcpy_*), so no other GPU runtime is involved. Import fails
+loudly when the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+PACKAGE = pathlib.Path(__file__).resolve().parent
+LIBRARY_PATH = PACKAGE / "libgsn_hip.so"
+
+PARAM_COUNT = 58
+POSITION, LOG_SCALE, ROTATION, OPACITY, SH = 0, 3, 6, 9, 10
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+class GsnError(RuntimeError):
+    """A C ABI entry point returned a non-zero status."""
+
+    def __init__(self, function: str, status: int, message: str):
+        super().__init__(f"{function} failed with status {status}: {message}")
+        self.status = status
+
+
+class Camera(ctypes.Structure):
+    """Byte-identical to gsn_camera / gs_native::CameraParams."""
+
+    _fields_ = [
+        ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+        ("R_c_w", ctypes.c_float * 9), ("t_c_w", ctypes.c_float * 3),
+        ("near_depth", ctypes.c_float), ("far_depth", ctypes.c_float),
+        ("width", ctypes.c_int), ("height", ctypes.c_int), ("pixel_offset", ctypes.c_int64),
+        ("tile_count_x", ctypes.c_int), ("tile_count_y", ctypes.c_int), ("tile_offset", ctypes.c_int),
+    ]
+
+    @classmethod
+    def make(cls, width: int, height: int, fx: float, fy: float, cx: float, cy: float,
+             rotation=None, translation=None, near: float = 0.2, far: float = FLT_MAX) -> "Camera":
+        camera = cls()
+        camera.fx, camera.fy, camera.cx, camera.cy = fx, fy, cx, cy
+        rotation = np.eye(3, dtype=np.float32) if rotation is None else np.asarray(rotation, np.float32)
+        translation = np.zeros(3, np.float32) if translation is None else np.asarray(translation, np.float32)
+        for k, value in enumerate(rotation.reshape(9)):
+            camera.R_c_w[k] = float(value)
+        for k, value in enumerate(translation.reshape(3)):
+            camera.t_c_w[k] = float(value)
+        camera.near_depth, camera.far_depth = near, far
+        camera.width, camera.height = width, height
+        camera.tile_count_x = (width + 15) // 16
+        camera.tile_count_y = (height + 15) // 16
+        return camera
+
+    @property
+    def tile_count(self) -> int:
+        return self.tile_count_x * self.tile_count_y
+
+    @property
+    def pixels(self) -> int:
+        return self.width * self.height
+
+    def rotation(self) -> np.ndarray:
+        return np.array(self.R_c_w[:], np.float32).reshape(3, 3)
+
+    def translation(self) -> np.ndarray:
+        return np.array(self.t_c_w[:], np.float32)
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_U64 = ctypes.c_uint64
+_I64 = ctypes.c_int64
+_CAM = ctypes.POINTER(Camera)
+
+_PROTOTYPES = {
+    "gsn_context_create": [ctypes.POINTER(_P)],
+    "gsn_context_destroy": [_P],
+    "gsn_context_synchronize": [_P],
+    "gsn_context_stream": [_P],
+    "gsn_context_scratch_bytes": [_P],
+    "gsn_status_string": [_I],
+    "gsn_last_error": [],
+    "gsn_device_count": [ctypes.POINTER(_I)],
+    "gsn_device_name": [ctypes.c_char_p, ctypes.c_size_t],
+    "gsn_device_malloc": [ctypes.POINTER(_P), ctypes.c_size_t],
+    "gsn_device_free": [_P],
+    "gsn_memcpy_to_device": [_P, _P, ctypes.c_size_t],
+    "gsn_memcpy_to_host": [_P, _P, ctypes.c_size_t],
+    "gsn_memcpy_device": [_P, _P, ctypes.c_size_t],
+    "gsn_memset_device": [_P, _I, ctypes.c_size_t],
+    "gsn_device_memory_info": [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+    "gsn_event_create": [ctypes.POINTER(_P)],
+    "gsn_event_destroy": [_P],
+    "gsn_event_record": [_P, _P],
+    "gsn_event_elapsed_ms": [_P, _P, ctypes.POINTER(_F)],
+    "gsn_project_gaussians": [_P, _P, _I, _CAM, _I, _P, _P, _P, _P, _P, _P],
+    "gsn_scan_record_offsets": [_P, _P, _I, ctypes.POINTER(_I)],
+    "gsn_build_tile_lists": [_P, _CAM, _I, _P, _P, _P, _I, _P, _P, _P],
+    "gsn_render_tiles": [_P, _CAM, _P, _P, _P, _P, _P, ctypes.POINTER(_F), _P],
+    "gsn_image_gradients": [_P, _I, _I, _P, _P, _P],
+    "gsn_image_metrics": [_P, _I, _I, _P, _P, ctypes.POINTER(ctypes.c_double)],
+    "gsn_gaussian_gradients": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _F, _F, _F, _P, _P],
+    "gsn_adam_update": [_P, _I, _P, _P, _P, _P, _F, _F, _F],
+    "gsn_remap_history": [_P, _P, _I, _I, _P, _P, _P, _P],
+    "gsn_zero_rows": [_P, _P, _P, _I, _I, _I],
+    "gsn_density_count": [_P, _P, _I, _P, _P, _P, _F, _F, _F, _F, _I, _F, _P, _P, ctypes.POINTER(ctypes.c_int32)],
+    "gsn_density_write": [_P, _P, _I, _P, _P, ctypes.POINTER(ctypes.c_int32), _U64, _U64, _P, _I, _P, _P, _P],
+    "gsn_density_normals": [_P, _U64, _U64, _I64, _P],
+    "gsn_density_accumulate": [_P, _P, _P, _I, _P, _P, _P],
+    "gsn_density_accumulate_observations": [_P, _P, _P, _P, _I, _I, _P, _P],
+    "gsn_density_reset_opacity": [_P, _P, _I, _I, _F],
+    "gsn_density_clear": [_P, _P, _P, _P, _I],
+    "gsn_knn3": [_P, _P, _I, _P, _P],
+    "gsn_selftest_wave_sum12": [_P, _P, _P],
+    "gsn_context_set_timing": [_P, _I],
+    "gsn_context_stage_times": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), _I],
+}
+
+STAGES = ("project", "scan", "depth_sort", "emit", "tile_sort", "ranges", "raster_forward", "image_gradients",
+          "raster_backward", "gaussian_gradients", "adam", "density", "knn")
+
+_RESTYPES = {
+    "gsn_context_destroy": None,
+    "gsn_context_stream": _P,
+    "gsn_context_scratch_bytes": ctypes.c_size_t,
+    "gsn_status_string": ctypes.c_char_p,
+    "gsn_last_error": ctypes.c_char_p,
+}
+
+
+def load_library(path: pathlib.Path = LIBRARY_PATH) -> ctypes.CDLL:
+    if not path.exists():
+        raise ImportError(f"{path} is not built: run __graft_entry__.build() (there is no CPU fallback)")
+    library = ctypes.CDLL(str(path))
+    for name, argtypes in _PROTOTYPES.items():
+        function = getattr(library, name)
+        function.argtypes = argtypes
+        function.restype = _RESTYPES.get(name, _I)
+    return library
+
+
+lib = load_library()
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_PROTOTYPES)
+
+
+def check(function: str, status: int) -> None:
+    if status != 0:
+        raise GsnError(function, status, lib.gsn_last_error().decode(errors="replace"))
+
+
+def call(function: str, *args) -> None:
+    check(function, getattr(lib, function)(*args))
+
+
+def device_count() -> int:
+    count = ctypes.c_int(0)
+    call("gsn_device_count", ctypes.byref(count))
+    return count.value
+
+
+def device_name() -> str:
+    buffer = ctypes.create_string_buffer(256)
+    call("gsn_device_name", buffer, 256)
+    return buffer.value.decode()
+
+
+class DeviceBuffer:
+    """Owning device allocation with a numpy dtype and shape."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self._pointer = ctypes.c_void_p(None)
+        if self.nbytes:
+            call("gsn_device_malloc", ctypes.byref(self._pointer), self.nbytes)
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._pointer
+
+    @classmethod
+    def from_numpy(cls, array: np.ndarray) -> "DeviceBuffer":
+        array = np.ascontiguousarray(array)
+        buffer = cls(array.shape, array.dtype)
+        buffer.upload(array)
+        return buffer
+
+    @classmethod
+    def zeros(cls, shape, dtype) -> "DeviceBuffer":
+        buffer = cls(shape, dtype)
+        if buffer.nbytes:
+            call("gsn_memset_device", buffer.ptr, 0, buffer.nbytes)
+        return buffer
+
+    def upload(self, array: np.ndarray) -> None:
+        array = np.ascontiguousarray(array, dtype=self.dtype)
+        if array.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch: {array.nbytes} != {self.nbytes}")
+        if self.nbytes:
+            call("gsn_memcpy_to_device", self.ptr, array.ctypes.data_as(ctypes.c_void_p), self.nbytes)
+
+    def numpy(self, count: int | None = None) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            call("gsn_memcpy_to_host", out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes)
+        return out
+
+    def free(self) -> None:
+        if self._pointer.value:
+            lib.gsn_device_free(self._pointer)
+            self._pointer = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A HIP stream plus the scratch memory of the kernels."""
+
+    def __init__(self):
+        self._handle = ctypes.c_void_p(None)
+        call("gsn_context_create", ctypes.byref(self._handle))
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._handle
+
+    def synchronize(self) -> None:
+        call("gsn_context_synchronize", self._handle)
+
+    def scratch_bytes(self) -> int:
+        return int(lib.gsn_context_scratch_bytes(self._handle))
+
+    def set_timing(self, enabled: bool) -> None:
+        call("gsn_context_set_timing", self._handle, int(enabled))
+
+    def stage_times(self) -> dict:
+        """{stage: (total milliseconds, timed launches)} since timing was enabled or last read."""
+        ms = (ctypes.c_double * len(STAGES))()
+        launches = (ctypes.c_int * len(STAGES))()
+        call("gsn_context_stage_times", self._handle, ms, launches, len(STAGES))
+        return {name: (ms[k], launches[k]) for k, name in enumerate(STAGES)}
+
+    def close(self) -> None:
+        if self._handle.value:
+            lib.gsn_context_destroy(self._handle)
+            self._handle = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self._handle = ctypes.c_void_p(None)
+        call("gsn_event_create", ctypes.byref(self._handle))
+
+    def record(self, context: Context) -> None:
+        call("gsn_event_record", context.handle, self._handle)
+
+    def elapsed_ms(self, end: "Event") -> float:
+        value = ctypes.c_float(0.0)
+        call("gsn_event_elapsed_ms", self._handle, end._handle, ctypes.byref(value))
+        return float(value.value)
+
+    def __del__(self):
+        try:
+            if self._handle.value:
+                lib.gsn_event_destroy(self._handle)
+        except Exception:
+            pass
+
+
+def _p(buffer) -> ctypes.c_void_p:
+    if buffer is None:
+        return ctypes.c_void_p(None)
+    if isinstance(buffer, DeviceBuffer):
+        return buffer.ptr
+    return ctypes.c_void_p(int(buffer))
+
+
+class Projection:
+    """Device outputs of projection + tile lists for one view (the Renderer's getters)."""
+
+    def __init__(self, count: int, camera: Camera):
+        self.count = count
+        self.camera = camera
+        n = max(count, 1)
+        self.means = DeviceBuffer((n, 2), np.float32)
+        self.conic_opacities = DeviceBuffer((n, 4), np.float32)
+        self.colors = DeviceBuffer((n, 3), np.float32)
+        self.depths = DeviceBuffer((n,), np.float32)
+        self.tile_rects = DeviceBuffer((n, 4), np.int32)
+        self.record_offsets = DeviceBuffer((count + 1,), np.int32)
+        self.tile_offsets = DeviceBuffer((camera.tile_count + 1,), np.int64)
+        self.record_count = 0
+        self.tile_gaussian_ids = DeviceBuffer((1,), np.int32)
+        self.sorted_keys = None
+
+
+def render(context: Context, params: DeviceBuffer, count: int, camera: Camera, background, image: DeviceBuffer,
+           sh_degree: int, projection: Projection | None = None, keep_keys: bool = False) -> Projection:
+    """Projection -> record scan -> tile lists -> raster forward (Renderer::Render)."""
+    if projection is None or projection.count != count:
+        projection = Projection(count, camera)
+    projection.camera = camera
+    if projection.tile_offsets.shape[0] < camera.tile_count + 1:
+        projection.tile_offsets = DeviceBuffer((camera.tile_count + 1,), np.int64)
+    h = context.handle
+    call("gsn_project_gaussians", h, _p(params), count, ctypes.byref(camera), sh_degree, _p(projection.means),
+         _p(projection.conic_opacities), _p(projection.colors), _p(projection.depths), _p(projection.tile_rects),
+         _p(projection.record_offsets))
+    records = ctypes.c_int(0)
+    call("gsn_scan_record_offsets", h, _p(projection.record_offsets), count, ctypes.byref(records))
+    projection.record_count = records.value
+    if projection.tile_gaussian_ids.shape[0] < max(records.value, 1):
+        projection.tile_gaussian_ids = DeviceBuffer((int(records.value * 1.25) + 1,), np.int32)
+    keys = None
+    if keep_keys:
+        projection.sorted_keys = DeviceBuffer((max(records.value, 1),), np.uint64)
+        keys = projection.sorted_keys
+    call("gsn_build_tile_lists", h, ctypes.byref(camera), count, _p(projection.tile_rects), _p(projection.depths),
+         _p(projection.record_offsets), records.value, _p(projection.tile_gaussian_ids), _p(projection.tile_offsets),
+         _p(keys))
+    bg = (ctypes.c_float * 3)(*[float(v) for v in background])
+    call("gsn_render_tiles", h, ctypes.byref(camera), _p(projection.tile_gaussian_ids), _p(projection.tile_offsets),
+         _p(projection.means), _p(projection.conic_opacities), _p(projection.colors), bg, _p(image))
+    return projection
+
+
+def image_gradients(context: Context, camera: Camera, image: DeviceBuffer, target: DeviceBuffer,
+                    out: DeviceBuffer) -> None:
+    call("gsn_image_gradients", context.handle, camera.width, camera.height, _p(image), _p(target), _p(out))
+
+
+def image_metrics(context: Context, width: int, height: int, image: DeviceBuffer, target: DeviceBuffer) -> dict:
+    out = (ctypes.c_double * 4)()
+    call("gsn_image_metrics", context.handle, width, height, _p(image), _p(target), out)
+    mse = out[3]
+    psnr = float("inf") if mse <= 0 else 10.0 * np.log10(1.0 / mse)
+    return {"loss": out[0], "l1": out[1], "ssim": out[2], "mse": mse, "psnr": psnr}
+
+
+def gaussian_gradients(context: Context, projection: Projection, params: DeviceBuffer, sh_degree: int,
+                       image: DeviceBuffer, image_grad: DeviceBuffer, gradients: DeviceBuffer | None,
+                       statistics: DeviceBuffer, absolute_statistics: DeviceBuffer,
+                       screen: DeviceBuffer | None = None) -> None:
+    p = projection
+    call("gsn_gaussian_gradients", context.handle, ctypes.byref(p.camera), p.count, sh_degree, _p(params),
+         _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets), _p(p.tile_rects),
+         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(gradients), _p(screen),
+         _p(statistics), _p(absolute_statistics))
+
+
+def gaussian_gradients_adam(context: Context, projection: Projection, params: DeviceBuffer, sh_degree: int,
+                            image: DeviceBuffer, image_grad: DeviceBuffer, first: DeviceBuffer, second: DeviceBuffer,
+                            position_lr: float, first_correction: float, second_correction: float,
+                            statistics: DeviceBuffer, absolute_statistics: DeviceBuffer) -> None:
+    p = projection
+    call("gsn_gaussian_gradients_adam", context.handle, ctypes.byref(p.camera), p.count, sh_degree, _p(params),
+         _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets), _p(p.tile_rects),
+         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(first), _p(second),
+         position_lr, first_correction, second_correction, _p(statistics), _p(absolute_statistics))
+
+
+def adam_corrections(step: int) -> tuple[float, float]:
+    """Bias corrections in FP64, rounded to FP32 (reference solver.cu:912-914)."""
+    first = np.float32(1.0 / (1.0 - 0.9 ** step))
+    second = np.float32(1.0 / (1.0 - 0.999 ** step))
+    return float(first), float(second)
+
+
+def adam_update(context: Context, count: int, gradients: DeviceBuffer, params: DeviceBuffer, first: DeviceBuffer,
+                second: DeviceBuffer, position_lr: float, step: int) -> None:
+    c1, c2 = adam_corrections(step)
+    call("gsn_adam_update", context.handle, count, _p(gradients), _p(params), _p(first), _p(second), position_lr, c1, c2)
+
+
+def knn3(context: Context, points: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = points.shape[0]
+    if n == 0:
+        return np.zeros((0, 3), np.int32), np.zeros(0, np.float32)
+    device_points = DeviceBuffer.from_numpy(points)
+    neighbors = DeviceBuffer((n, 3), np.int32)
+    distances = DeviceBuffer((n,), np.float32)
+    call("gsn_knn3", context.handle, _p(device_points), n, _p(neighbors), _p(distances))
+    context.synchronize()
+    return neighbors.numpy(), distances.numpy()

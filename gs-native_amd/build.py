@@ -1,0 +1,129 @@
+"""Build recipe for the native libraries of the gfx950 training path.
+
+libgsn_hip.so   HIP kernels + the C ABI declared in include/gsn_hip.h (hipcc, gfx950)
+libgs_native.so C++17 host classes mirroring the reference API (g++), linked to libgsn_hip.so
+
+Every call compiles from scratch; objects go to _build/ next to this file.
+"""
+from __future__ import annotations
+
+import concurrent.futures
+import os
+import pathlib
+import shutil
+import subprocess
+import sys
+
+PACKAGE = pathlib.Path(__file__).resolve().parent
+REPO = PACKAGE.parent
+CSRC = PACKAGE / "csrc"
+SRC = PACKAGE / "src"
+BUILD = PACKAGE / "_build"
+HIP_LIBRARY = PACKAGE / "libgsn_hip.so"
+CPP_LIBRARY = PACKAGE / "libgs_native.so"
+ROCM = pathlib.Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+HIPCC = str(ROCM / "bin" / "hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    # Packed FP32 instructions stay disabled (toolchain notes).
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-command-line-argument",
+]
+
+CXX_FLAGS = [
+    "-O2",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wextra",
+    "-D__HIP_PLATFORM_AMD__",
+    f"-I{ROCM / 'include'}",
+    f"-I{REPO / 'include'}",
+    f"-I{REPO / 'include' / 'gs_native'}",
+]
+
+
+def _run(command: list[str]) -> None:
+    result = subprocess.run(command, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if result.returncode != 0:
+        sys.stderr.write(result.stdout)
+        raise RuntimeError(f"command failed ({result.returncode}): {' '.join(command)}")
+    noisy = [line for line in result.stdout.splitlines() if "packed-fp32-ops" not in line and line.strip()]
+    if noisy:
+        sys.stderr.write("\n".join(noisy) + "\n")
+
+
+def _compile_hip(source: pathlib.Path) -> pathlib.Path:
+    target = BUILD / (source.stem + ".o")
+    _run([HIPCC, *HIP_FLAGS, "-c", str(source), "-o", str(target)])
+    return target
+
+
+def build_hip(jobs: int = 8) -> pathlib.Path:
+    BUILD.mkdir(exist_ok=True)
+    sources = sorted(CSRC.glob("*.hip"))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+        objects = list(pool.map(_compile_hip, sources))
+    temporary = HIP_LIBRARY.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(temporary), *map(str, objects)])
+    os.replace(temporary, HIP_LIBRARY)
+    return HIP_LIBRARY
+
+
+def build_cpp(jobs: int = 8) -> pathlib.Path | None:
+    sources = sorted(SRC.glob("*.cpp"))
+    if not sources:
+        return None
+    BUILD.mkdir(exist_ok=True)
+
+    def compile_one(source: pathlib.Path) -> pathlib.Path:
+        target = BUILD / (source.stem + ".cpp.o")
+        _run(["g++", *CXX_FLAGS, "-c", str(source), "-o", str(target)])
+        return target
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+        objects = list(pool.map(compile_one, sources))
+    temporary = CPP_LIBRARY.with_suffix(".so.tmp")
+    _run(["g++", "-shared", "-o", str(temporary), *map(str, objects), f"-L{PACKAGE}", "-lgsn_hip",
+          f"-L{ROCM / 'lib'}", "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM / 'lib'}"])
+    os.replace(temporary, CPP_LIBRARY)
+    return CPP_LIBRARY
+
+
+CPP_TEST_SOURCE = REPO / "tests" / "cpp" / "api_test.cpp"
+CPP_TEST_BINARY = REPO / "tests" / "cpp" / "api_test"
+
+
+def build_cpp_tests() -> pathlib.Path | None:
+    """The C++ API test program (runs on a GPU box; compiled here)."""
+    if not CPP_TEST_SOURCE.exists() or not CPP_LIBRARY.exists():
+        return None
+    temporary = CPP_TEST_BINARY.with_suffix(".tmp")
+    _run(["g++", *CXX_FLAGS, str(CPP_TEST_SOURCE), "-o", str(temporary), f"-L{PACKAGE}", "-lgs_native", "-lgsn_hip",
+          f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{PACKAGE}", "-Wl,-rpath,$ORIGIN/../../gs-native_amd",
+          f"-Wl,-rpath,{ROCM / 'lib'}"])
+    os.replace(temporary, CPP_TEST_BINARY)
+    return CPP_TEST_BINARY
+
+
+def build_all(jobs: int = 8) -> None:
+    build_hip(jobs)
+    build_cpp(jobs)
+    build_cpp_tests()
+
+
+def clean() -> None:
+    shutil.rmtree(BUILD, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    build_all()
+    print(HIP_LIBRARY)

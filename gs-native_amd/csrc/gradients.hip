@@ -1,0 +1,589 @@
+// Per-Gaussian gradient pass: reduces the backward partials of each Gaussian (contiguous, fixed
+// order), forms the projection statistics, applies the screen-to-parameter chain rule and either
+// stores the 58 parameter gradients or feeds them straight into Adam.
+//
+// Follows reference src/core/solver/solver.cu:
+//   CollectProjectionStatisticsKernel 468-490, MapGaussianGradientsKernel<D> 492-623
+//   (ComputeEquationRotation 185-221, ComputeEquationSh<D> 223-323),
+//   UpdateGaussiansKernel 651-689, RemapGaussianHistoryKernel 625-649.
+#include "gsn_internal.h"
+
+#include <cmath>
+
+namespace gsn {
+namespace {
+
+struct Mapped {
+    float grad[10];
+    float basis[16];
+    bool color_ok[3];
+};
+
+// Chain rule from the 9 screen-space gradients to position, log-scale, axis-angle and opacity
+// gradients; SH gradients are in[6 + c] * basis[k] for colour channels with value >= 0.
+template<int Degree>
+__device__ __forceinline__ void map_gaussian(const float* __restrict__ params, const Camera& cam, int id, int64_t n,
+    const float (&in)[11], Mapped& out) {
+    constexpr int s = (Degree + 1) * (Degree + 1);
+    float omega[3] = {params[(kRotationIndex + 0) * n + id], params[(kRotationIndex + 1) * n + id],
+        params[(kRotationIndex + 2) * n + id]};
+    float rotation[9];
+    rotation_from_axis_angle(omega[0], omega[1], omega[2], rotation);
+    float position[3], camera_rotation[9], scale_squared[3], world[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        world[r] = params[(kPositionIndex + r) * n + id];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        position[r] = cam.t_c_w[r];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            position[r] += cam.R_c_w[r * 3 + c] * world[c];
+            camera_rotation[r * 3 + c] = cam.R_c_w[r * 3] * rotation[c] + cam.R_c_w[r * 3 + 1] * rotation[3 + c] +
+                cam.R_c_w[r * 3 + 2] * rotation[6 + c];
+        }
+        float scale = __expf(params[(kLogScaleIndex + r) * n + id]);
+        scale_squared[r] = scale * scale;
+    }
+    float covariance[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                sum += camera_rotation[r * 3 + j] * scale_squared[j] * camera_rotation[c * 3 + j];
+            }
+            covariance[r * 3 + c] = sum;
+        }
+    }
+    float inverse_z = 1.0f / position[2];
+    float nx = position[0] * inverse_z, ny = position[1] * inverse_z;
+    float limit_x = kFovClampScale * 0.5f * static_cast<float>(cam.width) / cam.fx;
+    float limit_y = kFovClampScale * 0.5f * static_cast<float>(cam.height) / cam.fy;
+    float jx = fminf(fmaxf(nx, -limit_x), limit_x);
+    float jy = fminf(fmaxf(ny, -limit_y), limit_y);
+    float projection[6] = {cam.fx * inverse_z, 0.0f, -cam.fx * jx * inverse_z, 0.0f, cam.fy * inverse_z,
+        -cam.fy * jy * inverse_z};
+    float product[6], local_projection[6];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float p = 0.0f, l = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                p += projection[r * 3 + j] * covariance[j * 3 + c];
+                l += projection[r * 3 + j] * camera_rotation[j * 3 + c];
+            }
+            product[r * 3 + c] = p;
+            local_projection[r * 3 + c] = l;
+        }
+    }
+
+    // SH colour (unclamped) and its derivative with respect to the world position.
+    float direction[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        direction[a] = world[a];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            direction[a] += cam.R_c_w[b * 3 + a] * cam.t_c_w[b];
+        }
+    }
+    float inverse_length =
+        1.0f / sqrtf(direction[0] * direction[0] + direction[1] * direction[1] + direction[2] * direction[2]);
+    float x = direction[0] * inverse_length;
+    float y = direction[1] * inverse_length;
+    float z = direction[2] * inverse_length;
+    float d[48];
+#pragma unroll
+    for (int k = 0; k < 48; ++k) {
+        d[k] = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        out.basis[k] = 0.0f;
+    }
+    out.basis[0] = kShC0;
+    if (Degree >= 1) {
+        out.basis[1] = -kShC1 * y;
+        out.basis[2] = kShC1 * z;
+        out.basis[3] = -kShC1 * x;
+        d[4] = -kShC1;
+        d[8] = kShC1;
+        d[9] = -kShC1;
+    }
+    if (Degree >= 2) {
+        constexpr float c = kShC2_0, dd = kShC2_2, e = kShC2_4;
+        out.basis[4] = c * x * y;
+        out.basis[5] = -c * y * z;
+        out.basis[6] = dd * (2.0f * z * z - x * x - y * y);
+        out.basis[7] = -c * x * z;
+        out.basis[8] = e * (x * x - y * y);
+        d[12] = c * y;
+        d[13] = c * x;
+        d[16] = -c * z;
+        d[17] = -c * y;
+        d[18] = -2.0f * dd * x;
+        d[19] = -2.0f * dd * y;
+        d[20] = 4.0f * dd * z;
+        d[21] = -c * z;
+        d[23] = -c * x;
+        d[24] = 2.0f * e * x;
+        d[25] = -2.0f * e * y;
+    }
+    if (Degree >= 3) {
+        constexpr float a = kShC3_0, b = kShC3_1, c = kShC3_2, dd = kShC3_3, e = kShC3_5;
+        out.basis[9] = a * y * (3.0f * x * x - y * y);
+        out.basis[10] = b * x * y * z;
+        out.basis[11] = c * y * (4.0f * z * z - x * x - y * y);
+        out.basis[12] = dd * z * (2.0f * z * z - 3.0f * x * x - 3.0f * y * y);
+        out.basis[13] = c * x * (4.0f * z * z - x * x - y * y);
+        out.basis[14] = e * z * (x * x - y * y);
+        out.basis[15] = a * x * (x * x - 3.0f * y * y);
+        d[27] = 6.0f * a * x * y;
+        d[28] = 3.0f * a * (x * x - y * y);
+        d[30] = b * y * z;
+        d[31] = b * x * z;
+        d[32] = b * x * y;
+        d[33] = -2.0f * c * x * y;
+        d[34] = c * (4.0f * z * z - x * x - 3.0f * y * y);
+        d[35] = 8.0f * c * y * z;
+        d[36] = -6.0f * dd * x * z;
+        d[37] = -6.0f * dd * y * z;
+        d[38] = dd * (6.0f * z * z - 3.0f * x * x - 3.0f * y * y);
+        d[39] = c * (4.0f * z * z - 3.0f * x * x - y * y);
+        d[40] = -2.0f * c * x * y;
+        d[41] = 8.0f * c * x * z;
+        d[42] = 2.0f * e * x * z;
+        d[43] = -2.0f * e * y * z;
+        d[44] = e * (x * x - y * y);
+        d[45] = 3.0f * a * (x * x - y * y);
+        d[46] = -6.0f * a * x * y;
+    }
+    float color_derivatives[9];
+#pragma unroll
+    for (int channel = 0; channel < 3; ++channel) {
+        float value = 0.0f;
+        float derivative[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < s; ++j) {
+            float coefficient = params[(kShIndex + channel * 16 + j) * n + id];
+            value += coefficient * out.basis[j];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                derivative[a] += coefficient * d[j * 3 + a];
+            }
+        }
+        out.color_ok[channel] = value + 0.5f >= 0.0f;
+        float radial = derivative[0] * x + derivative[1] * y + derivative[2] * z;
+        color_derivatives[channel * 3 + 0] = (derivative[0] - radial * x) * inverse_length;
+        color_derivatives[channel * 3 + 1] = (derivative[1] - radial * y) * inverse_length;
+        color_derivatives[channel * 3 + 2] = (derivative[2] - radial * z) * inverse_length;
+    }
+
+    float right_gradient[3];
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        float dx = cam.R_c_w[axis], dy = cam.R_c_w[3 + axis], dz = cam.R_c_w[6 + axis];
+        float du = (dx - nx * dz) * inverse_z;
+        float dv = (dy - ny * dz) * inverse_z;
+        float djx = nx >= -limit_x && nx <= limit_x ? du : 0.0f;
+        float djy = ny >= -limit_y && ny <= limit_y ? dv : 0.0f;
+        float delta[6] = {-cam.fx * dz * inverse_z * inverse_z, 0.0f, -cam.fx * (djx - jx * dz * inverse_z) * inverse_z,
+            0.0f, -cam.fy * dz * inverse_z * inverse_z, -cam.fy * (djy - jy * dz * inverse_z) * inverse_z};
+        float daa = 0.0f, dab = 0.0f, ddd = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            daa += 2.0f * delta[j] * product[j];
+            dab += delta[j] * product[3 + j] + delta[3 + j] * product[j];
+            ddd += 2.0f * delta[3 + j] * product[3 + j];
+        }
+        float center_gradient =
+            in[0] * cam.fx * du + in[1] * cam.fy * dv + in[2] * daa + in[3] * dab + in[4] * ddd;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (out.color_ok[c]) {
+                center_gradient += in[6 + c] * color_derivatives[c * 3 + axis];
+            }
+        }
+        out.grad[kPositionIndex + axis] = center_gradient;
+        float a = local_projection[axis], b = local_projection[3 + axis];
+        out.grad[kLogScaleIndex + axis] = 2.0f * scale_squared[axis] * (in[2] * a * a + in[3] * a * b + in[4] * b * b);
+        int j = (axis + 1) % 3, k = (axis + 2) % 3;
+        float difference = scale_squared[j] - scale_squared[k];
+        right_gradient[axis] = difference *
+            (2.0f * in[2] * local_projection[j] * local_projection[k] +
+                in[3] * (local_projection[j] * local_projection[3 + k] + local_projection[k] * local_projection[3 + j]) +
+                2.0f * in[4] * local_projection[3 + j] * local_projection[3 + k]);
+    }
+    float theta_squared = omega[0] * omega[0] + omega[1] * omega[1] + omega[2] * omega[2];
+    float sa = 0.5f, sb = 1.0f / 6.0f;
+    if (theta_squared > 1e-4f) {
+        float theta = sqrtf(theta_squared);
+        float half_sinc = det_sinf(0.5f * theta) / (0.5f * theta);
+        sa = 0.5f * half_sinc * half_sinc;
+        sb = (theta - det_sinf(theta)) / (theta_squared * theta);
+    }
+    else {
+        sa -= theta_squared / 24.0f;
+        sb -= theta_squared / 120.0f;
+    }
+    float cross[3];
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        int j = (axis + 1) % 3, k = (axis + 2) % 3;
+        cross[axis] = omega[j] * right_gradient[k] - omega[k] * right_gradient[j];
+    }
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        int j = (axis + 1) % 3, k = (axis + 2) % 3;
+        out.grad[kRotationIndex + axis] =
+            right_gradient[axis] + sa * cross[axis] + sb * (omega[j] * cross[k] - omega[k] * cross[j]);
+    }
+    out.grad[kOpacityIndex] = in[5];
+}
+
+struct StoreSink {
+    float* gradients;
+    int64_t n;
+    int id;
+    __device__ __forceinline__ void operator()(int row, float g) const {
+        gradients[row * n + id] = g;
+    }
+};
+
+// Dense Adam step of one entry (reference solver.cu:666-688); no FP contraction so that the
+// result equals the host restatement bit for bit for equal inputs.
+__device__ __forceinline__ float learning_rate_for_row(int row, float position_learning_rate) {
+    if (row < kLogScaleIndex) {
+        return position_learning_rate;
+    }
+    if (row < kRotationIndex) {
+        return 0.005f;
+    }
+    if (row < kOpacityIndex) {
+        return 0.001f;
+    }
+    if (row < kShIndex) {
+        return 0.025f;
+    }
+    return (row - kShIndex) % 16 == 0 ? 0.0025f : 0.0025f / 20.0f;
+}
+
+__device__ __forceinline__ void adam_entry(float* params, float* first, float* second, int64_t index, float gradient,
+    float learning_rate, float first_correction, float second_correction) {
+#pragma clang fp contract(off)
+    float m = 0.9f * first[index] + 0.1f * gradient;
+    float v = 0.999f * second[index] + 0.001f * gradient * gradient;
+    first[index] = m;
+    second[index] = v;
+    params[index] -= learning_rate * (m * first_correction) / (sqrtf(v * second_correction) + 1e-15f);
+}
+
+struct AdamSink {
+    float* params;
+    float* first;
+    float* second;
+    int64_t n;
+    int id;
+    float position_learning_rate, first_correction, second_correction;
+    __device__ __forceinline__ void operator()(int row, float g) const {
+        adam_entry(params, first, second, row * n + id, g, learning_rate_for_row(row, position_learning_rate),
+            first_correction, second_correction);
+    }
+};
+
+template<int Degree, bool Fused>
+__global__ __launch_bounds__(256) void gaussian_gradients_kernel(const float* __restrict__ partials,
+    const int* __restrict__ record_offsets, int count, Camera cam, float* params, float* gradients, float* screen,
+    float* statistics, float* absolute_statistics, float* first, float* second, float position_learning_rate,
+    float first_correction, float second_correction) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count) {
+        return;
+    }
+    const int64_t n = count;
+    const int begin = record_offsets[id];
+    const int end = record_offsets[id + 1];
+    float in[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) {
+        in[k] = 0.0f;
+    }
+    for (int p = begin; p < end; ++p) {
+        const float4* record = reinterpret_cast<const float4*>(partials + static_cast<int64_t>(p) * kPartialFloats);
+        float4 a = record[0], b = record[1], c = record[2];
+        in[0] += a.x;
+        in[1] += a.y;
+        in[2] += a.z;
+        in[3] += a.w;
+        in[4] += b.x;
+        in[5] += b.y;
+        in[6] += b.z;
+        in[7] += b.w;
+        in[8] += c.x;
+        in[9] += c.y;
+        in[10] += c.z;
+    }
+    {
+        float x = in[0] * (0.5f * static_cast<float>(cam.width));
+        float y = in[1] * (0.5f * static_cast<float>(cam.height));
+        float ax = in[9] * (0.5f * static_cast<float>(cam.width));
+        float ay = in[10] * (0.5f * static_cast<float>(cam.height));
+        statistics[id] = sqrtf(x * x + y * y);
+        statistics[n + id] = end > begin ? 1.0f : 0.0f;
+        absolute_statistics[id] = sqrtf(ax * ax + ay * ay);
+        absolute_statistics[n + id] = in[10];
+    }
+    if (screen) {
+#pragma unroll
+        for (int k = 0; k < 11; ++k) {
+            screen[k * n + id] = in[k];
+        }
+    }
+    bool active = false;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        active = active || in[k] != 0.0f;
+    }
+    Mapped mapped;
+    if (active) {
+        map_gaussian<Degree>(params, cam, id, n, in, mapped);
+    }
+    constexpr int coefficients = (Degree + 1) * (Degree + 1);
+    auto emit = [&](auto& sink) {
+#pragma unroll
+        for (int row = 0; row < kParamCount; ++row) {
+            float g = 0.0f;
+            if (active) {
+                if (row < kShIndex) {
+                    g = mapped.grad[row];
+                }
+                else {
+                    int channel = (row - kShIndex) / 16;
+                    int k = (row - kShIndex) % 16;
+                    g = (k < coefficients && mapped.color_ok[channel]) ? in[6 + channel] * mapped.basis[k] : 0.0f;
+                }
+            }
+            sink(row, g);
+        }
+    };
+    if (Fused) {
+        AdamSink sink{params, first, second, n, id, position_learning_rate, first_correction, second_correction};
+        emit(sink);
+    }
+    else if (gradients) {
+        StoreSink sink{gradients, n, id};
+        emit(sink);
+    }
+}
+
+__global__ __launch_bounds__(256) void adam_kernel(const float* __restrict__ gradients, float* params, float* first,
+    float* second, int count, float position_learning_rate, float first_correction, float second_correction) {
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count) {
+        return;
+    }
+    int row = blockIdx.y;
+    int64_t index = static_cast<int64_t>(row) * count + id;
+    adam_entry(params, first, second, index, gradients[index], learning_rate_for_row(row, position_learning_rate),
+        first_correction, second_correction);
+}
+
+__global__ __launch_bounds__(256) void remap_history_kernel(const int* parent_ids, const float* previous_first,
+    const float* previous_second, float* first, float* second, int previous_count, int count) {
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count) {
+        return;
+    }
+    int row = blockIdx.y;
+    int parent = parent_ids ? parent_ids[id] : -1;
+    int64_t index = static_cast<int64_t>(row) * count + id;
+    if (parent >= 0 && parent < previous_count) {
+        int64_t previous = static_cast<int64_t>(row) * previous_count + parent;
+        first[index] = previous_first[previous];
+        second[index] = previous_second[previous];
+    }
+    else {
+        first[index] = 0.0f;
+        second[index] = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_rows_kernel(float* first, float* second, int64_t begin, int64_t total) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < total) {
+        first[begin + i] = 0.0f;
+        second[begin + i] = 0.0f;
+    }
+}
+
+template<bool Fused>
+int launch_gradients(gsn_context* context, const Camera& cam, int count, int sh_degree, const float* partials,
+    const int* record_offsets, float* params, float* gradients, float* screen, float* statistics,
+    float* absolute_statistics, float* first, float* second, float lr, float c1, float c2) {
+    unsigned grid = grid_for(count, 256);
+    hipStream_t stream = context->stream;
+    StageScope scope(context, kStageGaussianGradients);
+    switch (clamp_sh_degree(sh_degree)) {
+    case 0:
+        gaussian_gradients_kernel<0, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, cam, params,
+            gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
+        break;
+    case 1:
+        gaussian_gradients_kernel<1, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, cam, params,
+            gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
+        break;
+    case 2:
+        gaussian_gradients_kernel<2, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, cam, params,
+            gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
+        break;
+    default:
+        gaussian_gradients_kernel<3, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, cam, params,
+            gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
+        break;
+    }
+    return check_launch("gaussian_gradients_kernel");
+}
+
+int validate_gradient_inputs(const gsn_context* context, const gsn_camera* camera, int count, const void* params,
+    const int* ids, const int64_t* tile_offsets, int record_count, const int* record_offsets, const int* rects,
+    const float* means, const float* conics, const float* colors, const float* image, const float* image_gradients,
+    const float* statistics, const float* absolute_statistics) {
+    if (!context || !camera || count < 0 || record_count < 0) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gaussian gradients: invalid arguments");
+    }
+    if (camera->width <= 0 || camera->height <= 0 || camera->tile_count_x <= 0 || camera->tile_count_y <= 0) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gaussian gradients: invalid camera");
+    }
+    if (count > 0 && (!params || !tile_offsets || !record_offsets || !rects || !means || !conics || !colors || !image ||
+                         !image_gradients || !statistics || !absolute_statistics)) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gaussian gradients: null buffer");
+    }
+    if (record_count > 0 && !ids) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gaussian gradients: null tile ids");
+    }
+    return GSN_OK;
+}
+
+} // namespace
+} // namespace gsn
+
+extern "C" int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* camera, int count, int sh_degree,
+    const float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
+    const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
+    const float* colors, const float* image, const float* image_gradients, float* gradients, float* screen_gradients,
+    float* statistics, float* absolute_statistics) {
+    using namespace gsn;
+    GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
+        record_offsets, tile_rects, means, conic_opacities, colors, image, image_gradients, statistics,
+        absolute_statistics));
+    if (count == 0) {
+        return GSN_OK;
+    }
+    Camera cam;
+    memcpy(&cam, camera, sizeof(cam));
+    float* partials = static_cast<float*>(
+        scratch(context, kSlotPartials, static_cast<size_t>(record_count > 0 ? record_count : 1) * kPartialFloats * 4));
+    if (!partials) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    if (record_count > 0) {
+        GSN_TRY(launch_splat_backward(context, cam, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors,
+            tile_rects, record_offsets, image, image_gradients, partials));
+    }
+    return launch_gradients<false>(context, cam, count, sh_degree, partials, record_offsets, const_cast<float*>(params),
+        gradients, screen_gradients, statistics, absolute_statistics, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
+}
+
+extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, int count, int sh_degree,
+    float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
+    const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
+    const float* colors, const float* image, const float* image_gradients, float* first_moments,
+    float* second_moments, float position_learning_rate, float first_correction, float second_correction,
+    float* statistics, float* absolute_statistics) {
+    using namespace gsn;
+    GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
+        record_offsets, tile_rects, means, conic_opacities, colors, image, image_gradients, statistics,
+        absolute_statistics));
+    if (count == 0) {
+        return GSN_OK;
+    }
+    if (!first_moments || !second_moments || !(position_learning_rate >= 0.0f) || !std::isfinite(position_learning_rate)) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_gaussian_gradients_adam: invalid optimiser arguments");
+    }
+    Camera cam;
+    memcpy(&cam, camera, sizeof(cam));
+    float* partials = static_cast<float*>(
+        scratch(context, kSlotPartials, static_cast<size_t>(record_count > 0 ? record_count : 1) * kPartialFloats * 4));
+    if (!partials) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    if (record_count > 0) {
+        GSN_TRY(launch_splat_backward(context, cam, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors,
+            tile_rects, record_offsets, image, image_gradients, partials));
+    }
+    return launch_gradients<true>(context, cam, count, sh_degree, partials, record_offsets, params, nullptr, nullptr,
+        statistics, absolute_statistics, first_moments, second_moments, position_learning_rate, first_correction,
+        second_correction);
+}
+
+extern "C" int gsn_adam_update(gsn_context* context, int count, const float* gradients, float* params,
+    float* first_moments, float* second_moments, float position_learning_rate, float first_correction,
+    float second_correction) {
+    using namespace gsn;
+    if (!context || count < 0 || !(position_learning_rate >= 0.0f) || !std::isfinite(position_learning_rate)) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_adam_update: invalid arguments");
+    }
+    if (count == 0) {
+        return GSN_OK;
+    }
+    if (!gradients || !params || !first_moments || !second_moments) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_adam_update: null buffer");
+    }
+    dim3 grid(grid_for(count, 256), kParamCount);
+    StageScope scope(context, kStageAdam);
+    adam_kernel<<<grid, 256, 0, context->stream>>>(gradients, params, first_moments, second_moments, count,
+        position_learning_rate, first_correction, second_correction);
+    return check_launch("adam_kernel");
+}
+
+extern "C" int gsn_remap_history(gsn_context* context, const int* parent_ids, int previous_count, int count,
+    const float* previous_first, const float* previous_second, float* first, float* second) {
+    using namespace gsn;
+    if (!context || count < 0 || previous_count < 0) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_remap_history: invalid arguments");
+    }
+    if (count == 0) {
+        return GSN_OK;
+    }
+    if (!first || !second || (previous_count > 0 && parent_ids && (!previous_first || !previous_second))) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_remap_history: null buffer");
+    }
+    dim3 grid(grid_for(count, 256), kParamCount);
+    StageScope scope(context, kStageDensity);
+    remap_history_kernel<<<grid, 256, 0, context->stream>>>(parent_ids, previous_first, previous_second, first, second,
+        previous_first ? previous_count : 0, count);
+    return check_launch("remap_history_kernel");
+}
+
+extern "C" int gsn_zero_rows(gsn_context* context, float* first, float* second, int stride, int row, int rows) {
+    using namespace gsn;
+    if (!context || stride < 0 || row < 0 || rows < 0 || row + rows > kParamCount) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_zero_rows: invalid arguments");
+    }
+    int64_t total = static_cast<int64_t>(stride) * rows;
+    if (total == 0) {
+        return GSN_OK;
+    }
+    if (!first || !second) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_zero_rows: null buffer");
+    }
+    zero_rows_kernel<<<grid_for(total, 256), 256, 0, context->stream>>>(first, second, static_cast<int64_t>(row) * stride,
+        total);
+    return check_launch("zero_rows_kernel");
+}

@@ -1,0 +1,132 @@
+// Internal declarations shared by the HIP translation units of libgsn_hip.so.
+#pragma once
+
+#include "gsn_common.h"
+#include "../../include/gsn_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+
+// Scratch slots owned by a context. Each slot grows on demand and is reused across calls.
+enum ScratchSlot {
+    kSlotSortKeysA = 0,
+    kSlotSortKeysB,
+    kSlotSortValuesA,
+    kSlotSortValuesB,
+    kSlotSortHistogram,
+    kSlotScanBlocks,
+    kSlotDepthKeys,
+    kSlotDepthIds,
+    kSlotDepthOffsets,
+    kSlotDepthSortedIds,
+    kSlotDepthCounts,
+    kSlotRecordTiles,
+    kSlotRecordIds,
+    kSlotPartials,
+    kSlotReduce,
+    kSlotKnnKeys,
+    kSlotKnnIds,
+    kSlotKnnNodes,
+    kSlotKnnPoints,
+    kSlotMisc,
+    kSlotCount
+};
+
+// Stages timed with HIP events when a context's stage timing is enabled (order = gsn_stage).
+enum Stage {
+    kStageProject = 0,
+    kStageScan,
+    kStageDepthSort,
+    kStageEmit,
+    kStageTileSort,
+    kStageRanges,
+    kStageRasterForward,
+    kStageImageGradients,
+    kStageRasterBackward,
+    kStageGaussianGradients,
+    kStageAdam,
+    kStageDensity,
+    kStageKnn,
+    kStageCount
+};
+
+struct gsn_context {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int cu_count = 0;
+    void* slots[kSlotCount] = {};
+    size_t slot_bytes[kSlotCount] = {};
+    void* pinned = nullptr;  // small pinned host buffer for scalar readbacks
+    // Stage timing: event pairs recorded around kernel launches while enabled.
+    bool timing = false;
+    hipEvent_t* events = nullptr;
+    int* event_stage = nullptr;
+    int event_capacity = 0;  // number of pairs
+    int event_used = 0;
+};
+
+namespace gsn {
+
+// Records an event pair around the launches in its scope when timing is enabled.
+class StageScope {
+public:
+    StageScope(gsn_context* context, Stage stage);
+    ~StageScope();
+
+private:
+    gsn_context* context_;
+    int pair_;
+};
+
+} // namespace gsn
+
+namespace gsn {
+
+// Records a failure message for gsn_last_error() and returns the status.
+int fail(int status, const char* format, ...);
+int check_hip(hipError_t error, const char* what);
+int check_launch(const char* kernel);
+// Returns a device buffer of at least `bytes` for `slot`, growing it if needed (no copy).
+void* scratch(gsn_context* context, ScratchSlot slot, size_t bytes);
+
+// Device-wide exclusive prefix scan of int / int4-like arrays (scan.hip).
+struct Int4 {
+    int x, y, z, w;
+};
+int exclusive_scan_int(gsn_context* context, const int* input, int* output, int64_t n);
+int exclusive_scan_int4(gsn_context* context, const Int4* input, Int4* output, int64_t n);
+
+// Stable LSD radix sort of (uint32 key, uint32 value) pairs over key bits [0, bits)
+// (radix_sort.hip). Input buffers are consumed. The result is written to keys_out/values_out
+// (either may be the same buffer as an input only if it is not referenced by a later pass;
+// callers pass distinct buffers). keys_out may be null when only values are needed.
+int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int64_t n,
+    int bits, uint32_t* keys_out, uint32_t* values_out);
+
+// Floats per (tile, Gaussian) backward partial: dmean2D(2), dSigma2D(3), dlogit, dcolor(3),
+// |dmean2D|(2), pad.
+constexpr int kPartialFloats = 12;
+
+// Tile rasteriser backward writing per-record partials (raster_backward.hip).
+int launch_splat_backward(gsn_context* context, const Camera& cam, const int* ids, const int64_t* tile_offsets,
+    const float* means, const float* conics, const float* colors, const int* rects, const int* record_offsets,
+    const float* image, const float* image_gradients, float* partials);
+
+inline unsigned grid_for(int64_t n, int block) {
+    return static_cast<unsigned>((n + block - 1) / block);
+}
+
+} // namespace gsn
+
+#define GSN_TRY(expr)                 \
+    do {                              \
+        int gsn_status__ = (expr);    \
+        if (gsn_status__ != GSN_OK) { \
+            return gsn_status__;      \
+        }                             \
+    } while (0)
+
+#define GSN_HIP(expr) GSN_TRY(::gsn::check_hip((expr), #expr))

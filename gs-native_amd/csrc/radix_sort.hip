@@ -1,0 +1,199 @@
+// Stable LSD radix sort of (uint32 key, uint32 value) pairs for wave64 / gfx950.
+//
+// Each pass (<= 8 digit bits) runs three kernels over 4096-element tiles:
+//   1. radix_histogram: per-tile digit histogram in LDS, written bin-major [bin][tile].
+//   2. exclusive scan of the histogram table (scan.hip) -> global base of every (bin, tile).
+//   3. radix_scatter: stable in-tile ranking with wave ballots (digit match masks), local
+//      reordering through LDS, then a write-out in which consecutive lanes store consecutive
+//      positions of a digit run.
+// Stability: a tile is split into four 1024-element wave chunks processed in index order, and
+// ranks are offset by the digit counts of earlier waves, so equal digits keep input order.
+#include "gsn_internal.h"
+
+namespace gsn {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
+constexpr int kWaves = kSortThreads / 64;
+constexpr int kWaveChunk = kSortTile / kWaves;        // 1024
+
+__global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(const uint32_t* keys, int64_t n, int shift,
+    int bins, int tiles, int* histogram) {
+    __shared__ int counts[256];
+    int t = threadIdx.x;
+    counts[t] = 0;
+    __syncthreads();
+    int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    uint32_t mask = static_cast<uint32_t>(bins - 1);
+#pragma unroll 4
+    for (int i = 0; i < kSortItems; ++i) {
+        int64_t index = base + i * kSortThreads + t;
+        if (index < n) {
+            uint32_t digit = (keys[index] >> shift) & mask;
+            atomicAdd(&counts[digit], 1);
+        }
+    }
+    __syncthreads();
+    if (t < bins) {
+        histogram[static_cast<int64_t>(t) * tiles + blockIdx.x] = counts[t];
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint32_t* keys_in, const uint32_t* values_in,
+    int64_t n, int shift, int bits, int tiles, const int* offsets, uint32_t* keys_out, uint32_t* values_out) {
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_values[kSortTile];
+    __shared__ int s_wave_count[kWaves][256];
+    __shared__ int s_digit_start[256];
+    __shared__ int s_global[256];
+    __shared__ int s_scan[256];
+
+    const int t = threadIdx.x;
+    const int wave = t >> 6;
+    const int lane = t & 63;
+    const int bins = 1 << bits;
+    const uint32_t mask = static_cast<uint32_t>(bins - 1);
+    const int64_t tile_base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    const int valid = static_cast<int>(n - tile_base < kSortTile ? n - tile_base : kSortTile);
+    const uint64_t lanemask_lt = (1ull << lane) - 1ull;
+
+    for (int d = lane; d < 256; d += 64) {
+        s_wave_count[wave][d] = 0;
+    }
+    __syncthreads();
+
+    uint32_t key_reg[kSortItems];
+    uint32_t value_reg[kSortItems];
+    int rank_reg[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        int local = wave * kWaveChunk + j * 64 + lane;
+        bool in_range = local < valid;
+        uint32_t key = in_range ? keys_in[tile_base + local] : 0xffffffffu;
+        uint32_t value = in_range ? values_in[tile_base + local] : 0u;
+        // Out-of-range slots take the largest digit so that they rank after every valid element.
+        uint32_t digit = in_range ? ((key >> shift) & mask) : mask;
+        uint64_t peers = ~0ull;
+        for (int b = 0; b < bits; ++b) {
+            uint64_t vote = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? vote : ~vote;
+        }
+        int before = __popcll(peers & lanemask_lt);
+        int count = s_wave_count[wave][digit];
+        rank_reg[j] = count + before;
+        if (before == 0) {
+            s_wave_count[wave][digit] = count + __popcll(peers);
+        }
+        // Keeps the next round's counter reads after this round's leader writes.
+        __builtin_amdgcn_wave_barrier();
+        key_reg[j] = key;
+        value_reg[j] = value;
+    }
+    __syncthreads();
+
+    // Per digit: exclusive prefix across waves (in place) and the tile total.
+    int digit_total = 0;
+    if (t < bins) {
+        int running = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            int c = s_wave_count[w][t];
+            s_wave_count[w][t] = running;
+            running += c;
+        }
+        digit_total = running;
+        s_global[t] = offsets[static_cast<int64_t>(t) * tiles + blockIdx.x];
+    }
+    s_scan[t] = t < bins ? digit_total : 0;
+    __syncthreads();
+    for (int offset = 1; offset < 256; offset <<= 1) {
+        int other = t >= offset ? s_scan[t - offset] : 0;
+        __syncthreads();
+        s_scan[t] += other;
+        __syncthreads();
+    }
+    s_digit_start[t] = s_scan[t] - (t < bins ? digit_total : 0);
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        int local = wave * kWaveChunk + j * 64 + lane;
+        uint32_t digit = local < valid ? ((key_reg[j] >> shift) & mask) : mask;
+        int position = s_digit_start[digit] + s_wave_count[wave][digit] + rank_reg[j];
+        s_keys[position] = key_reg[j];
+        s_values[position] = value_reg[j];
+    }
+    __syncthreads();
+
+    for (int i = t; i < valid; i += kSortThreads) {
+        uint32_t key = s_keys[i];
+        uint32_t digit = (key >> shift) & mask;
+        int64_t destination = static_cast<int64_t>(s_global[digit]) + (i - s_digit_start[digit]);
+        if (keys_out) {
+            keys_out[destination] = key;
+        }
+        values_out[destination] = s_values[i];
+    }
+}
+
+} // namespace
+
+int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int64_t n, int bits,
+    uint32_t* keys_out, uint32_t* values_out) {
+    if (n <= 0) {
+        return GSN_OK;
+    }
+    if (n > 0x7fffffffll) {
+        return fail(GSN_ERROR_CAPACITY, "radix_sort_pairs: %lld elements exceed int32 positions", static_cast<long long>(n));
+    }
+    if (bits < 1) {
+        bits = 1;
+    }
+    if (bits > 32) {
+        bits = 32;
+    }
+    int passes = (bits + 7) / 8;
+    int tiles = static_cast<int>((n + kSortTile - 1) / kSortTile);
+    size_t elements = static_cast<size_t>(n);
+    uint32_t* scratch_keys = static_cast<uint32_t*>(scratch(context, kSlotSortKeysA, elements * sizeof(uint32_t)));
+    uint32_t* scratch_values = static_cast<uint32_t*>(scratch(context, kSlotSortValuesA, elements * sizeof(uint32_t)));
+    int* histogram = static_cast<int*>(scratch(context, kSlotSortHistogram, static_cast<size_t>(tiles) * 256 * sizeof(int)));
+    if (!scratch_keys || !scratch_values || !histogram) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    uint32_t* final_keys = keys_out;
+    if (!final_keys) {
+        final_keys = static_cast<uint32_t*>(scratch(context, kSlotSortKeysB, elements * sizeof(uint32_t)));
+        if (!final_keys) {
+            return GSN_ERROR_OUT_OF_MEMORY;
+        }
+    }
+    // Pass p reads src and writes dst; the last pass writes the caller's outputs (which must not
+    // alias the inputs), earlier passes alternate scratch, input, scratch, ... so that no pass
+    // reads and writes the same buffer.
+    uint32_t* src_keys = keys;
+    uint32_t* src_values = values;
+    int shift = 0;
+    for (int pass = 0; pass < passes; ++pass) {
+        int remaining = bits - shift;
+        int digit_bits = (remaining + (passes - pass) - 1) / (passes - pass);
+        int bins = 1 << digit_bits;
+        bool last = pass == passes - 1;
+        bool to_scratch = (pass % 2) == 0;
+        uint32_t* dst_keys = last ? final_keys : (to_scratch ? scratch_keys : keys);
+        uint32_t* dst_values = last ? values_out : (to_scratch ? scratch_values : values);
+        radix_histogram_kernel<<<tiles, kSortThreads, 0, context->stream>>>(src_keys, n, shift, bins, tiles, histogram);
+        GSN_TRY(check_launch("radix_histogram_kernel"));
+        GSN_TRY(exclusive_scan_int(context, histogram, histogram, static_cast<int64_t>(bins) * tiles));
+        radix_scatter_kernel<<<tiles, kSortThreads, 0, context->stream>>>(src_keys, src_values, n, shift, digit_bits, tiles,
+            histogram, dst_keys, dst_values);
+        GSN_TRY(check_launch("radix_scatter_kernel"));
+        src_keys = dst_keys;
+        src_values = dst_values;
+        shift += digit_bits;
+    }
+    return GSN_OK;
+}
+
+} // namespace gsn

@@ -1,0 +1,133 @@
+// Tile rasteriser forward (reference src/core/render/renderer.cu:273-338).
+// Per pixel, front to back: skip power > 0; alpha = min(0.99, o * e^power); skip alpha < 1/255;
+// if T * (1 - alpha) < 1e-4 the pixel is done without blending that splat; otherwise
+// C += c * alpha * T and T *= 1 - alpha. Output = C + T * background.
+#include "raster_common.h"
+
+namespace gsn {
+namespace {
+
+__global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restrict__ ids,
+    const int64_t* __restrict__ tile_offsets, const float2* __restrict__ means, const float4* __restrict__ conics,
+    const float* __restrict__ colors, Camera cam, float* __restrict__ image, float bg_r, float bg_g, float bg_b,
+    int tile_count) {
+    const int tile = xcd_remap(blockIdx.x, tile_count);
+    const int lane = threadIdx.x;
+    const int x0 = (tile % cam.tile_count_x) * kTileSize;
+    const int y0 = (tile / cam.tile_count_x) * kTileSize;
+    const int px = x0 + (lane & 15);
+    const int py = y0 + (lane >> 4);
+    const float fx = static_cast<float>(px);
+    float fy[4], T[4], cr[4], cg[4], cb[4];
+    bool done[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        int y = py + 4 * b;
+        fy[b] = static_cast<float>(y);
+        done[b] = !(px < cam.width && y < cam.height);
+        T[b] = 1.0f;
+        cr[b] = 0.0f;
+        cg[b] = 0.0f;
+        cb[b] = 0.0f;
+    }
+    const int64_t begin = tile_offsets[tile];
+    const int64_t end = tile_offsets[tile + 1];
+    for (int64_t base = begin; base < end; base += 64) {
+        int active = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (__ballot(!done[b])) {
+                active |= 1 << b;
+            }
+        }
+        if (active == 0) {
+            break;
+        }
+        const int64_t record = base + lane;
+        float mx = 0.0f, my = 0.0f, sr = 0.0f, sg = 0.0f, sb = 0.0f;
+        float4 conic = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        int mask = 0;
+        if (record < end) {
+            int id = ids[record];
+            float2 mean = means[id];
+            conic = conics[id];
+            const float* color = colors + static_cast<int64_t>(id) * 3;
+            sr = color[0];
+            sg = color[1];
+            sb = color[2];
+            mx = mean.x;
+            my = mean.y;
+            mask = band_mask(mx, my, conic, x0, y0);
+        }
+        uint64_t pending = __ballot((mask & active) != 0);
+        while (pending) {
+            const int j = __builtin_ctzll(pending);
+            pending &= pending - 1;
+            const int smask = readlane_i(mask, j) & active;
+            const float sx = readlane_f(mx, j);
+            const float sy = readlane_f(my, j);
+            const float ca = readlane_f(conic.x, j);
+            const float cbv = readlane_f(conic.y, j);
+            const float cc = readlane_f(conic.z, j);
+            const float op = readlane_f(conic.w, j);
+            const float r = readlane_f(sr, j);
+            const float g = readlane_f(sg, j);
+            const float bl = readlane_f(sb, j);
+            const float dx = sx - fx;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if (smask & (1 << b)) {
+                    float dy = sy - fy[b];
+                    float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbv * dx * dy;
+                    float alpha = fminf(kAlphaMax, op * __expf(power));
+                    bool take = !done[b] && !(power > 0.0f) && alpha >= kAlphaMin;
+                    float next = T[b] * (1.0f - alpha);
+                    bool finish = take && next < kTransmittanceMin;
+                    bool blend = take && !finish;
+                    float w = alpha * T[b];
+                    if (blend) {
+                        cr[b] += r * w;
+                        cg[b] += g * w;
+                        cb[b] += bl * w;
+                        T[b] = next;
+                    }
+                    done[b] = done[b] || finish;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        int y = py + 4 * b;
+        if (px < cam.width && y < cam.height) {
+            float* pixel = image + (static_cast<int64_t>(y) * cam.width + px) * 3;
+            pixel[0] = cr[b] + T[b] * bg_r;
+            pixel[1] = cg[b] + T[b] * bg_g;
+            pixel[2] = cb[b] + T[b] * bg_b;
+        }
+    }
+}
+
+} // namespace
+} // namespace gsn
+
+extern "C" int gsn_render_tiles(gsn_context* context, const gsn_camera* camera, const int* tile_gaussian_ids,
+    const int64_t* tile_offsets, const float* means, const float* conic_opacities, const float* colors,
+    const float* background, float* image) {
+    using namespace gsn;
+    if (!context || !camera || !tile_offsets || !background || !image || camera->width <= 0 || camera->height <= 0 ||
+        camera->tile_count_x <= 0 || camera->tile_count_y <= 0) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_render_tiles: invalid arguments");
+    }
+    int64_t tiles = static_cast<int64_t>(camera->tile_count_x) * camera->tile_count_y;
+    if (tiles >= 0x7fffffff) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_render_tiles: too many tiles");
+    }
+    Camera cam;
+    memcpy(&cam, camera, sizeof(cam));
+    StageScope scope(context, kStageRasterForward);
+    raster_forward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(tile_gaussian_ids, tile_offsets,
+        reinterpret_cast<const float2*>(means), reinterpret_cast<const float4*>(conic_opacities), colors, cam, image,
+        background[0], background[1], background[2], static_cast<int>(tiles));
+    return check_launch("raster_forward_kernel");
+}

@@ -1,0 +1,279 @@
+// Fused L1 + SSIM image gradient (reference src/core/solver/solver.cu:26-168) and the loss /
+// metric reduction the reference lacks.
+//
+// L = 0.8 * mean|I - T| + 0.2 * (1 - mean SSIM) over 3*W*H values; SSIM uses the separable
+// 11-tap Gaussian window (sigma 1.5, normalised) with zero padding, C1 = 1e-4, C2 = 9e-4.
+// One workgroup computes a 32x16 output tile of one channel: a 10-pixel halo is staged in LDS,
+// the five window moments are filtered horizontally then vertically, the per-position SSIM
+// derivative coefficients (a, b, c) are formed, and the adjoint filter is applied (again
+// separably) to give dL/dI = (0.8 sign(I - T) - 0.2 (Wa + T Wb + 2 I Wc)) / (3 W H).
+#include "gsn_internal.h"
+
+#include <cmath>
+#include <vector>
+
+namespace gsn {
+namespace {
+
+constexpr int kTw = 32;
+constexpr int kTh = 16;
+constexpr int kIw = kTw + 20;  // input columns (10-pixel halo each side)
+constexpr int kIh = kTh + 20;  // input rows
+constexpr int kMw = kTw + 10;  // moment columns (output +- 5)
+constexpr int kMh = kTh + 10;  // moment rows after the vertical pass
+constexpr int kThreads = 256;
+
+struct Window {
+    float w[11];
+};
+
+Window make_window() {
+    // Normalised Gaussian, sigma = 1.5, radius 5 (cv::getGaussianKernel(11, 1.5)).
+    double values[11], sum = 0.0;
+    for (int i = 0; i < 11; ++i) {
+        double x = i - 5;
+        values[i] = std::exp(-x * x / (2.0 * 1.5 * 1.5));
+        sum += values[i];
+    }
+    Window window;
+    for (int i = 0; i < 11; ++i) {
+        window.w[i] = static_cast<float>(values[i] / sum);
+    }
+    return window;
+}
+
+struct SsimTile {
+    float in[2 * kIh * kIw];
+    float h[5 * kIh * kMw];
+};
+
+// Stages 1-3 for one channel: loads the input with halo, filters the five moments, and calls
+// `at(my, mx, moments, inside)` for every moment position of the (kTh+10) x (kTw+10) region.
+template<typename Visit>
+__device__ __forceinline__ void ssim_moments(SsimTile& s, const Window& win, const float* image, const uint8_t* target,
+    int width, int height, int channel, int origin_x, int origin_y, Visit at) {
+    const int t = threadIdx.x;
+    for (int idx = t; idx < kIh * kIw; idx += kThreads) {
+        int iy = idx / kIw, ix = idx - iy * kIw;
+        int gx = origin_x - 10 + ix, gy = origin_y - 10 + iy;
+        bool valid = gx >= 0 && gy >= 0 && gx < width && gy < height;
+        int64_t index = (static_cast<int64_t>(gy) * width + gx) * 3 + channel;
+        s.in[idx] = valid ? image[index] : 0.0f;
+        s.in[kIh * kIw + idx] = valid ? static_cast<float>(target[index]) * (1.0f / 255.0f) : 0.0f;
+    }
+    __syncthreads();
+    for (int idx = t; idx < kIh * kMw; idx += kThreads) {
+        int iy = idx / kMw, mx = idx - iy * kMw;
+        float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f;
+#pragma unroll
+        for (int dx = 0; dx < 11; ++dx) {
+            float w = win.w[dx];
+            float a = s.in[iy * kIw + mx + dx];
+            float b = s.in[kIh * kIw + iy * kIw + mx + dx];
+            m0 += w * a;
+            m1 += w * b;
+            m2 += w * a * a;
+            m3 += w * b * b;
+            m4 += w * a * b;
+        }
+        s.h[0 * kIh * kMw + idx] = m0;
+        s.h[1 * kIh * kMw + idx] = m1;
+        s.h[2 * kIh * kMw + idx] = m2;
+        s.h[3 * kIh * kMw + idx] = m3;
+        s.h[4 * kIh * kMw + idx] = m4;
+    }
+    __syncthreads();
+    for (int idx = t; idx < kMh * kMw; idx += kThreads) {
+        int my = idx / kMw, mx = idx - my * kMw;
+        float m[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int dy = 0; dy < 11; ++dy) {
+            float w = win.w[dy];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                m[k] += w * s.h[k * kIh * kMw + (my + dy) * kMw + mx];
+            }
+        }
+        int hx = origin_x - 5 + mx, hy = origin_y - 5 + my;
+        bool inside = hx >= 0 && hy >= 0 && hx < width && hy < height;
+        at(my, mx, m, inside);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void image_gradients_kernel(const float* __restrict__ image,
+    const uint8_t* __restrict__ target, float* __restrict__ gradients, int width, int height, Window win) {
+    __shared__ SsimTile s;
+    const int channel = blockIdx.z;
+    const int origin_x = blockIdx.x * kTw;
+    const int origin_y = blockIdx.y * kTh;
+    // Coefficient maps reuse the input region once the moments exist; the horizontal adjoint
+    // reuses the moment region.
+    float* coef = s.in;  // [3][kMh][kMw]
+    ssim_moments(s, win, image, target, width, height, channel, origin_x, origin_y,
+        [&](int my, int mx, const float (&m)[5], bool inside) {
+            float a = 0.0f, b = 0.0f, c = 0.0f;
+            if (inside) {
+                float mu_a = m[0], mu_b = m[1];
+                float inverse_d = 1.0f / (mu_a * mu_a + mu_b * mu_b + 0.0001f);
+                float inverse_e = 1.0f / (m[2] - mu_a * mu_a + m[3] - mu_b * mu_b + 0.0009f);
+                float luminance = (2.0f * mu_a * mu_b + 0.0001f) * inverse_d;
+                float contrast = (2.0f * (m[4] - mu_a * mu_b) + 0.0009f) * inverse_e;
+                float mean_derivative = 2.0f * (mu_b - mu_a * luminance) * inverse_d * contrast;
+                b = 2.0f * luminance * inverse_e;
+                c = -luminance * contrast * inverse_e;
+                a = mean_derivative - mu_b * b - 2.0f * mu_a * c;
+            }
+            // All reads of s.in finished before the barrier that precedes this stage.
+            coef[0 * kMh * kMw + my * kMw + mx] = a;
+            coef[1 * kMh * kMw + my * kMw + mx] = b;
+            coef[2 * kMh * kMw + my * kMw + mx] = c;
+        });
+    __syncthreads();
+    float* adjoint = s.h;  // [3][kMh][kTw]
+    const int t = threadIdx.x;
+    for (int idx = t; idx < kMh * kTw; idx += kThreads) {
+        int my = idx / kTw, ox = idx - my * kTw;
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+#pragma unroll
+        for (int dx = 0; dx < 11; ++dx) {
+            float w = win.w[dx];
+            f0 += w * coef[0 * kMh * kMw + my * kMw + ox + dx];
+            f1 += w * coef[1 * kMh * kMw + my * kMw + ox + dx];
+            f2 += w * coef[2 * kMh * kMw + my * kMw + ox + dx];
+        }
+        adjoint[0 * kMh * kTw + idx] = f0;
+        adjoint[1 * kMh * kTw + idx] = f1;
+        adjoint[2 * kMh * kTw + idx] = f2;
+    }
+    __syncthreads();
+    const float normalization = 1.0f / (3.0f * static_cast<float>(width) * static_cast<float>(height));
+    for (int idx = t; idx < kTh * kTw; idx += kThreads) {
+        int oy = idx / kTw, ox = idx - oy * kTw;
+        int x = origin_x + ox, y = origin_y + oy;
+        if (x >= width || y >= height) {
+            continue;
+        }
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < 11; ++dy) {
+            float w = win.w[dy];
+            f0 += w * adjoint[0 * kMh * kTw + (oy + dy) * kTw + ox];
+            f1 += w * adjoint[1 * kMh * kTw + (oy + dy) * kTw + ox];
+            f2 += w * adjoint[2 * kMh * kTw + (oy + dy) * kTw + ox];
+        }
+        int64_t pixel = (static_cast<int64_t>(y) * width + x) * 3 + channel;
+        float render = image[pixel];
+        float reference = static_cast<float>(target[pixel]) * (1.0f / 255.0f);
+        float residual = render - reference;
+        float l1_gradient = static_cast<float>(residual > 0.0f) - static_cast<float>(residual < 0.0f);
+        float ssim_gradient = -(f0 + reference * f1 + 2.0f * render * f2);
+        gradients[pixel] = (0.8f * l1_gradient + 0.2f * ssim_gradient) * normalization;
+    }
+}
+
+// Per-block sums of (|I - T|, SSIM, (clamp(I, 0, 1) - T)^2) over the block's output pixels.
+__global__ __launch_bounds__(kThreads) void image_metrics_kernel(const float* __restrict__ image,
+    const uint8_t* __restrict__ target, double* __restrict__ block_sums, int width, int height, Window win) {
+    __shared__ SsimTile s;
+    __shared__ double reduce[3][kThreads];
+    const int channel = blockIdx.z;
+    const int origin_x = blockIdx.x * kTw;
+    const int origin_y = blockIdx.y * kTh;
+    float ssim_sum = 0.0f;
+    ssim_moments(s, win, image, target, width, height, channel, origin_x, origin_y,
+        [&](int my, int mx, const float (&m)[5], bool inside) {
+            int ox = mx - 5, oy = my - 5;
+            if (!inside || ox < 0 || oy < 0 || ox >= kTw || oy >= kTh) {
+                return;
+            }
+            float mu_a = m[0], mu_b = m[1];
+            float luminance = (2.0f * mu_a * mu_b + 0.0001f) / (mu_a * mu_a + mu_b * mu_b + 0.0001f);
+            float contrast = (2.0f * (m[4] - mu_a * mu_b) + 0.0009f) / (m[2] - mu_a * mu_a + m[3] - mu_b * mu_b + 0.0009f);
+            ssim_sum += luminance * contrast;
+        });
+    float l1_sum = 0.0f, squared_sum = 0.0f;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < kTh * kTw; idx += kThreads) {
+        int oy = idx / kTw, ox = idx - oy * kTw;
+        int x = origin_x + ox, y = origin_y + oy;
+        if (x >= width || y >= height) {
+            continue;
+        }
+        int64_t pixel = (static_cast<int64_t>(y) * width + x) * 3 + channel;
+        float render = image[pixel];
+        float reference = static_cast<float>(target[pixel]) * (1.0f / 255.0f);
+        l1_sum += fabsf(render - reference);
+        float clamped = fminf(fmaxf(render, 0.0f), 1.0f) - reference;
+        squared_sum += clamped * clamped;
+    }
+    reduce[0][t] = l1_sum;
+    reduce[1][t] = ssim_sum;
+    reduce[2][t] = squared_sum;
+    __syncthreads();
+    for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
+        if (t < stride) {
+            for (int k = 0; k < 3; ++k) {
+                reduce[k][t] += reduce[k][t + stride];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        int64_t block = (static_cast<int64_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        for (int k = 0; k < 3; ++k) {
+            block_sums[block * 3 + k] = reduce[k][0];
+        }
+    }
+}
+
+const Window& window() {
+    static const Window w = make_window();
+    return w;
+}
+
+} // namespace
+} // namespace gsn
+
+extern "C" int gsn_image_gradients(gsn_context* context, int width, int height, const float* image,
+    const uint8_t* target, float* image_gradients) {
+    using namespace gsn;
+    if (!context || width <= 0 || height <= 0 || !image || !target || !image_gradients) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_image_gradients: invalid arguments");
+    }
+    dim3 grid(grid_for(width, kTw), grid_for(height, kTh), 3);
+    StageScope scope(context, kStageImageGradients);
+    image_gradients_kernel<<<grid, kThreads, 0, context->stream>>>(image, target, image_gradients, width, height,
+        window());
+    return check_launch("image_gradients_kernel");
+}
+
+extern "C" int gsn_image_metrics(gsn_context* context, int width, int height, const float* image,
+    const uint8_t* target, double* out) {
+    using namespace gsn;
+    if (!context || width <= 0 || height <= 0 || !image || !target || !out) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_image_metrics: invalid arguments");
+    }
+    dim3 grid(grid_for(width, kTw), grid_for(height, kTh), 3);
+    size_t blocks = static_cast<size_t>(grid.x) * grid.y * grid.z;
+    double* sums = static_cast<double*>(scratch(context, kSlotReduce, blocks * 3 * sizeof(double)));
+    if (!sums) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    image_metrics_kernel<<<grid, kThreads, 0, context->stream>>>(image, target, sums, width, height, window());
+    GSN_TRY(check_launch("image_metrics_kernel"));
+    std::vector<double> host(blocks * 3);
+    GSN_HIP(hipMemcpyAsync(host.data(), sums, host.size() * sizeof(double), hipMemcpyDeviceToHost, context->stream));
+    GSN_HIP(hipStreamSynchronize(context->stream));
+    double l1 = 0.0, ssim = 0.0, squared = 0.0;
+    for (size_t b = 0; b < blocks; ++b) {
+        l1 += host[b * 3 + 0];
+        ssim += host[b * 3 + 1];
+        squared += host[b * 3 + 2];
+    }
+    double values = 3.0 * static_cast<double>(width) * height;
+    out[1] = l1 / values;
+    out[2] = ssim / values;
+    out[0] = 0.8 * out[1] + 0.2 * (1.0 - out[2]);
+    out[3] = squared / values;
+    return GSN_OK;
+}

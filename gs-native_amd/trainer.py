@@ -1,0 +1,153 @@
+"""Training loop over the C ABI: the per-iteration schedule of the reference's
+GaussianSplatting::Train (src/core/gaussian_splatting.cpp:130-198, 262-287).
+
+Per iteration: camera from an epoch shuffle, SH degree min(3, it/1000), position learning rate
+extent * exp-interp(1.6e-4 -> 1.6e-6 over 30000), render, L1+SSIM gradient, fused
+gradients + Adam; while it < 15000: statistics accumulation, density control every 100 iterations
+after 500 (max screen size 20 after 3000, absolute threshold 8e-4, seed 0) and an opacity cap
+every 500 (0.01 every 3000, else 0.8) with the opacity moments cleared. Targets stay resident in
+device memory; the only host synchronisation per iteration is the record-count readback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import pathlib
+import sys
+
+import numpy as np
+
+if __package__:
+    from . import DeviceBuffer, Projection, adam_corrections, call, gaussian_gradients_adam, image_gradients, render
+    from . import _p
+else:  # loaded by path
+    _pkg = sys.modules["gs_native_amd"]
+    DeviceBuffer, Projection, adam_corrections, call = _pkg.DeviceBuffer, _pkg.Projection, _pkg.adam_corrections, _pkg.call
+    gaussian_gradients_adam, image_gradients, render, _p = (_pkg.gaussian_gradients_adam, _pkg.image_gradients,
+                                                             _pkg.render, _pkg._p)
+
+
+class Trainer:
+    def __init__(self, context, params: np.ndarray, cameras: list, targets: list, scene_extent: float,
+                 background=(0.0, 0.0, 0.0), max_count: int | None = None, completed: int = 0, seed: int = 0):
+        self.context = context
+        self.count = int(params.shape[1])
+        self.params = DeviceBuffer.from_numpy(np.ascontiguousarray(params, np.float32))
+        self.first = DeviceBuffer.zeros((58, self.count), np.float32)
+        self.second = DeviceBuffer.zeros((58, self.count), np.float32)
+        self.statistics = DeviceBuffer.zeros((2, self.count), np.float32)
+        self.absolute_statistics = DeviceBuffer.zeros((2, self.count), np.float32)
+        self.sums = DeviceBuffer.zeros((self.count,), np.float32)
+        self.absolute_sums = DeviceBuffer.zeros((self.count,), np.float32)
+        self.observations = DeviceBuffer.zeros((self.count,), np.uint32)
+        self.cameras = cameras
+        self.targets = [t if isinstance(t, DeviceBuffer) else DeviceBuffer.from_numpy(np.ascontiguousarray(t, np.uint8))
+                        for t in targets]
+        pixels = max(c.width * c.height for c in cameras)
+        self.image = DeviceBuffer((pixels, 3), np.float32)
+        self.image_grad = DeviceBuffer((pixels, 3), np.float32)
+        self.scene_extent = float(scene_extent)
+        self.background = background
+        self.max_count = max_count
+        self.completed = completed
+        self.adam_step = 0
+        self.sh_degree = 0
+        self.projection = None
+        self.rng = np.random.default_rng(seed)
+        self.order = np.arange(len(cameras))
+        self.cursor = len(cameras)
+        self.density_counter = 0
+        self.last_camera = -1
+        self.history = []
+
+    # -- schedule ------------------------------------------------------------------------------
+    def position_learning_rate(self, iteration: int) -> float:
+        fraction = np.float32(min(1.0, iteration / 30000.0))
+        value = np.float32(math.exp(math.log(np.float32(0.00016)) * (1.0 - fraction)
+                                    + math.log(np.float32(0.0000016)) * fraction))
+        return float(np.float32(self.scene_extent) * value)
+
+    def next_camera(self) -> int:
+        if self.cursor >= len(self.order):
+            self.rng.shuffle(self.order)
+            self.cursor = 0
+        index = int(self.order[self.cursor])
+        self.cursor += 1
+        return index
+
+    def iteration(self, camera_index: int | None = None) -> int:
+        index = self.next_camera() if camera_index is None else camera_index
+        self.last_camera = index
+        camera = self.cameras[index]
+        iteration = self.completed + 1
+        self.sh_degree = min(3, iteration // 1000)
+        lr = self.position_learning_rate(iteration)
+        self.projection = render(self.context, self.params, self.count, camera, self.background, self.image,
+                                 self.sh_degree, self.projection if self.projection and self.projection.count == self.count else None)
+        image_gradients(self.context, camera, self.image, self.targets[index], self.image_grad)
+        self.adam_step += 1
+        c1, c2 = adam_corrections(self.adam_step)
+        gaussian_gradients_adam(self.context, self.projection, self.params, self.sh_degree, self.image,
+                                self.image_grad, self.first, self.second, lr, c1, c2, self.statistics,
+                                self.absolute_statistics)
+        self.completed = iteration
+        records = self.projection.record_count
+        if self.completed < 15000:
+            call("gsn_density_accumulate", self.context.handle, self.statistics.ptr, self.absolute_statistics.ptr,
+                 self.count, self.sums.ptr, self.absolute_sums.ptr, self.observations.ptr)
+            if self.completed > 500 and self.completed % 100 == 0:
+                self.control_density()
+            if self.completed % 500 == 0:
+                self.reset_opacity(0.01 if self.completed % 3000 == 0 else 0.8)
+        return records
+
+    # -- density -------------------------------------------------------------------------------
+    def reset_opacity(self, cap: float) -> None:
+        logit = float(np.float32(math.log(np.float32(cap) / (np.float32(1.0) - np.float32(cap)))))
+        call("gsn_density_reset_opacity", self.context.handle, self.params.ptr, self.count, self.count, logit)
+        call("gsn_zero_rows", self.context.handle, self.first.ptr, self.second.ptr, self.count, 9, 1)
+
+    def control_density(self, gradient_threshold=0.0002, percent_dense=0.01, min_opacity=0.005,
+                        absolute_threshold=0.0008) -> tuple[int, int]:
+        count = self.count
+        max_screen = 20.0 if self.completed > 3000 else 0.0
+        decisions = DeviceBuffer((count + 1, 4), np.int32)
+        offsets = DeviceBuffer((count + 1, 4), np.int32)
+        totals = (ctypes.c_int32 * 4)()
+        split_threshold = float(np.float32(percent_dense) * np.float32(self.scene_extent))
+        world = float(np.float32(0.1) * np.float32(self.scene_extent))
+
+        def count_changes(threshold, absolute):
+            call("gsn_density_count", self.context.handle, self.params.ptr, count, self.sums.ptr,
+                 self.absolute_sums.ptr, self.observations.ptr, threshold, split_threshold, min_opacity, world,
+                 int(max_screen > 0), absolute, decisions.ptr, offsets.ptr, totals)
+            t = list(totals)
+            return t, t[0] + t[1] + 2 * t[3]
+
+        t, out_count = count_changes(gradient_threshold, absolute_threshold)
+        if self.max_count is not None and out_count > self.max_count:
+            # Capacity cap: no clones or splits this round, pruning only.
+            t, out_count = count_changes(float("inf"), float("inf"))
+        params = DeviceBuffer((58, max(out_count, 1)), np.float32)
+        history = DeviceBuffer((max(out_count, 1),), np.int32)
+        call("gsn_density_write", self.context.handle, self.params.ptr, count, decisions.ptr, offsets.ptr, totals, 0,
+             self.density_counter, params.ptr, out_count, history.ptr, None, None)
+        self.density_counter += 1
+        first = DeviceBuffer((58, max(out_count, 1)), np.float32)
+        second = DeviceBuffer((58, max(out_count, 1)), np.float32)
+        call("gsn_remap_history", self.context.handle, history.ptr, count, out_count, self.first.ptr, self.second.ptr,
+             first.ptr, second.ptr)
+        self.params, self.first, self.second = params, first, second
+        self.count = out_count
+        self.statistics = DeviceBuffer.zeros((2, max(out_count, 1)), np.float32)
+        self.absolute_statistics = DeviceBuffer.zeros((2, max(out_count, 1)), np.float32)
+        self.sums = DeviceBuffer.zeros((max(out_count, 1),), np.float32)
+        self.absolute_sums = DeviceBuffer.zeros((max(out_count, 1),), np.float32)
+        self.observations = DeviceBuffer.zeros((max(out_count, 1),), np.uint32)
+        self.projection = None
+        changes = (t[1] + 2 * t[3], count - t[0])
+        self.history.append((self.completed, count, out_count, changes))
+        return changes
+
+    def params_host(self) -> np.ndarray:
+        return self.params.numpy()[:, : self.count]

@@ -1,0 +1,62 @@
+// GaussianSplatting (reference src/core/gaussian_splatting.h:14-53): initialisation from a point
+// cloud (3-NN scales), the per-iteration training schedule, rendering and PLY export.
+#pragma once
+
+#include "core/density/density_controller.h"
+#include "core/render/renderer.h"
+#include "core/solver/solver.h"
+#include "core/types/scene.h"
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+namespace gs_native {
+
+class GaussianSplatting {
+public:
+    GaussianSplatting() = default;
+    ~GaussianSplatting();
+
+    GaussianSplatting(const GaussianSplatting&) = delete;
+    GaussianSplatting& operator=(const GaussianSplatting&) = delete;
+
+    bool Initialize(const Scene& scene);
+
+    bool Train(int iterations);
+
+    // Renders into image (float RGB, unclamped), like the reference's CV_32FC3 output.
+    bool Render(const Camera& camera, ImageF& image);
+
+    bool Save(const std::string& path);
+
+    int GetGaussianCount() const {
+        return gaussians_.gaussian_count_;
+    }
+
+private:
+    bool InitializeGaussians();
+    bool ControlDensity();
+
+    Scene scene_;
+    GaussianBuffers gaussians_;
+
+    Renderer renderer_;
+    Solver solver_;
+    DensityController density_;
+
+    float* image_ = nullptr;
+    uint8_t* target_ = nullptr;
+    std::vector<uint8_t*> targets_;
+    size_t pixel_capacity_ = 0;
+
+    std::vector<int> camera_order_;
+    int camera_cursor_ = 0;
+
+    int completed_ = 0;
+    int sh_degree_ = 0;
+    float scene_extent_ = 0.0f;
+    float background_[3] = {0.0f, 0.0f, 0.0f};
+};
+
+} // namespace gs_native

@@ -1,0 +1,254 @@
+/*
+ * gsn_hip.h - C ABI of the MI355X (gfx950) Gaussian-splatting training hot path.
+ *
+ * Every entry point takes an opaque context (one HIP stream plus the scratch memory the
+ * kernels need), device pointers, POD structs and sizes, and returns an int status
+ * (GSN_OK == 0). No C++ types cross this boundary. All work is enqueued on the context's
+ * stream; entry points that hand a value back to the host say so and synchronise that stream.
+ *
+ * Layouts follow the reference (Naive-q/GS-native):
+ *   params            float[58][N]  rows: 0-2 position, 3-5 log-scale, 6-8 axis-angle,
+ *                                   9 opacity logit, 10+16c+j SH channel c coeff j
+ *                                   (src/core/types/buffers.h:10-15)
+ *   means             float[N][2]   pixel-space centres          (renderer.h:25)
+ *   conic_opacities   float[N][4]   (A, B, C, sigmoid(opacity))   (renderer.h:29)
+ *   colors            float[N][3]   SH colour, clamped below at 0 (renderer.h:33)
+ *   depths            float[N]      camera-space z
+ *   tile_rects        int[N][4]     (x0, y0, x1, y1) tile rectangle, exclusive
+ *   record_offsets    int[N+1]      records per Gaussian, then their exclusive prefix
+ *   tile_gaussian_ids int[R]        Gaussian id per record, tile-major then depth
+ *   tile_offsets      int64[T+1]    [begin, end) of each tile's records
+ *   images            float[H][W][3] interleaved RGB; targets uint8[H][W][3]
+ *
+ * Each declaration names the reference function it replaces (file:line in the reference).
+ */
+#ifndef GSN_HIP_H
+#define GSN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GSN_OK = 0,
+    GSN_ERROR_INVALID_ARGUMENT = 1,
+    GSN_ERROR_HIP = 2,
+    GSN_ERROR_OUT_OF_MEMORY = 3,
+    GSN_ERROR_CAPACITY = 4,
+};
+
+/* Byte-identical to gs_native::CameraParams (src/core/types/buffers.h:17-28). */
+typedef struct gsn_camera {
+    float fx, fy, cx, cy;
+    float R_c_w[9];
+    float t_c_w[3];
+    float near_depth, far_depth;
+    int width, height;
+    int64_t pixel_offset;
+    int tile_count_x, tile_count_y;
+    int tile_offset;
+} gsn_camera;
+
+typedef struct gsn_context gsn_context;
+
+/* ---- context, status, device plumbing -------------------------------------------------- */
+
+/* Creates a context with its own HIP stream (default priority, blocking w.r.t. stream 0). */
+int gsn_context_create(gsn_context** context);
+void gsn_context_destroy(gsn_context* context);
+int gsn_context_synchronize(gsn_context* context);
+/* Returns the context's hipStream_t. */
+void* gsn_context_stream(gsn_context* context);
+/* Bytes of scratch memory currently held by the context. */
+size_t gsn_context_scratch_bytes(gsn_context* context);
+const char* gsn_status_string(int status);
+/* Message describing the most recent failure on the calling thread ("" if none). */
+const char* gsn_last_error(void);
+
+int gsn_device_count(int* count);
+int gsn_device_name(char* name, size_t capacity);
+int gsn_device_malloc(void** pointer, size_t bytes);
+int gsn_device_free(void* pointer);
+int gsn_memcpy_to_device(void* destination, const void* source, size_t bytes);
+int gsn_memcpy_to_host(void* destination, const void* source, size_t bytes);
+int gsn_memcpy_device(void* destination, const void* source, size_t bytes);
+int gsn_memset_device(void* destination, int value, size_t bytes);
+/* Peak bytes reported by hipMemGetInfo deltas are not tracked; this reports free/total. */
+int gsn_device_memory_info(size_t* free_bytes, size_t* total_bytes);
+
+/* Per-stage kernel timing (no counterpart in the reference, which times only whole calls with
+ * cv::TickMeter). While enabled, every stage records a HIP event pair on the context stream;
+ * gsn_context_stage_times synchronises, returns the summed milliseconds and the number of timed
+ * launches of each stage, and clears the record. Stage order:
+ *   0 project, 1 scan, 2 depth sort, 3 record emit, 4 tile sort, 5 tile ranges,
+ *   6 raster forward, 7 image gradients, 8 raster backward, 9 per-Gaussian gradients (+Adam),
+ *   10 Adam, 11 density control, 12 KNN. */
+#define GSN_STAGE_COUNT 13
+int gsn_context_set_timing(gsn_context* context, int enabled);
+int gsn_context_stage_times(gsn_context* context, double* milliseconds, int* launches, int stages);
+
+/* Diagnostic: the wave64 reduction used by the raster backward, on one wave.
+ * input float[64][12] -> output float[64][3]; lane l receives the sums over all lanes of
+ * fields 3*(l/16) .. 3*(l/16)+2. No counterpart in the reference. */
+int gsn_selftest_wave_sum12(gsn_context* context, const float* input, float* output);
+
+/* Events on the context stream, for timing. */
+int gsn_event_create(void** event);
+int gsn_event_destroy(void* event);
+int gsn_event_record(gsn_context* context, void* event);
+int gsn_event_elapsed_ms(void* start, void* end, float* milliseconds);
+
+/* ---- renderer ---------------------------------------------------------------------------- */
+
+/* Projection + SH colour + tile rectangle + record count per Gaussian.
+ * Replaces Renderer::ProjectGaussians / ProjectGaussiansKernel (renderer.cu:374-444, 108-221).
+ * record_offsets[N+1] receives per-Gaussian record counts (0 when culled) and 0 at [N];
+ * other outputs of culled Gaussians are left unchanged. */
+int gsn_project_gaussians(gsn_context* context, const float* params, int count,
+    const gsn_camera* camera, int sh_degree, float* means, float* conic_opacities,
+    float* colors, float* depths, int* tile_rects, int* record_offsets);
+
+/* In-place exclusive scan of record_offsets[N+1]; the total R is copied to *record_count
+ * (synchronises the stream). Replaces the DeviceScan + readback at renderer.cu:465-487. */
+int gsn_scan_record_offsets(gsn_context* context, int* record_offsets, int count,
+    int* record_count);
+
+/* Tile keys, stable sort by (tile, depth bits), per-tile ranges.
+ * Replaces WriteTileRecordsKernel, DeviceRadixSort::SortPairs and IdentifyTileOffsetsKernel
+ * (renderer.cu:223-271, 534-569). Output order equals a stable sort of the reference keys
+ * (tile << 32 | float_bits(depth)) over records enumerated in Gaussian-id order.
+ * sorted_keys[R] is optional (NULL to skip). */
+int gsn_build_tile_lists(gsn_context* context, const gsn_camera* camera, int count,
+    const int* tile_rects, const float* depths, const int* record_offsets, int record_count,
+    int* tile_gaussian_ids, int64_t* tile_offsets, uint64_t* sorted_keys);
+
+/* Tile rasteriser forward, front-to-back alpha blending.
+ * Replaces Renderer::RenderTiles / RenderTilesKernel (renderer.cu:572-590, 273-338).
+ * background points to 3 host floats. */
+int gsn_render_tiles(gsn_context* context, const gsn_camera* camera,
+    const int* tile_gaussian_ids, const int64_t* tile_offsets, const float* means,
+    const float* conic_opacities, const float* colors, const float* background, float* image);
+
+/* ---- solver ------------------------------------------------------------------------------ */
+
+/* dL/dimage for L = 0.8*mean|I-T| + 0.2*(1-mean SSIM), 11-tap sigma 1.5 window, zero padding.
+ * Replaces Solver::ComputeImageGradients / ComputeImageGradientsKernel (solver.cu:735-759, 26-168). */
+int gsn_image_gradients(gsn_context* context, int width, int height, const float* image,
+    const uint8_t* target, float* image_gradients);
+
+/* Loss value and image metrics (no counterpart in the reference, which never evaluates the
+ * loss on the training path, solver.cu:162-166). out[0] = 0.8*L1 + 0.2*(1-SSIM), out[1] = mean
+ * |I-T|, out[2] = mean SSIM, out[3] = mean squared error of the image clamped to [0,1] (for
+ * PSNR). Synchronises the stream. */
+int gsn_image_metrics(gsn_context* context, int width, int height, const float* image,
+    const uint8_t* target, double* out);
+
+/* Splat backward (recomputes transmittance front to back from the final colour), projection
+ * statistics and the screen-to-parameter chain rule.
+ * Replaces ComputeSplatGradientsKernel, CollectProjectionStatisticsKernel and
+ * MapGaussianGradientsKernel<D> (solver.cu:837-899, 325-623).
+ * gradients[58][N]: parameter gradients (rows of unobserved Gaussians are zero).
+ * screen_gradients[11][N] (optional): dmean2D(2), dconic(3), dlogit, dcolor(3), |dmean2D|(2).
+ * statistics[2][N]: (|dmean2D| scaled by W/2,H/2; observed flag).
+ * absolute_statistics[2][N]: row 0 = norm of the per-pixel absolute dmean2D sums. */
+int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* camera, int count,
+    int sh_degree, const float* params, const int* tile_gaussian_ids,
+    const int64_t* tile_offsets, int record_count, const int* record_offsets,
+    const int* tile_rects, const float* means, const float* conic_opacities,
+    const float* colors, const float* image, const float* image_gradients, float* gradients,
+    float* screen_gradients, float* statistics, float* absolute_statistics);
+
+/* Same as gsn_gaussian_gradients followed by gsn_adam_update, fused so that parameter
+ * gradients never go to memory. Replaces Solver::Iterate's gradient + update stages
+ * (solver.cu:702-720, 837-929). */
+int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, int count,
+    int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
+    int record_count, const int* record_offsets, const int* tile_rects, const float* means,
+    const float* conic_opacities, const float* colors, const float* image,
+    const float* image_gradients, float* first_moments, float* second_moments,
+    float position_learning_rate, float first_correction, float second_correction,
+    float* statistics, float* absolute_statistics);
+
+/* Dense Adam over all 58*N entries with the reference's per-group learning rates.
+ * Replaces UpdateGaussiansKernel (solver.cu:651-689, 901-929). */
+int gsn_adam_update(gsn_context* context, int count, const float* gradients, float* params,
+    float* first_moments, float* second_moments, float position_learning_rate,
+    float first_correction, float second_correction);
+
+/* Gathers Adam moments by parent id (parent < 0 or >= previous_count -> zero).
+ * parent_ids may be NULL (all zero). Replaces RemapGaussianHistoryKernel (solver.cu:625-649). */
+int gsn_remap_history(gsn_context* context, const int* parent_ids, int previous_count,
+    int count, const float* previous_first, const float* previous_second, float* first,
+    float* second);
+
+/* Zeroes rows [row, row+rows) of two [58][stride] moment buffers (opacity history reset,
+ * solver.cu:722-733). */
+int gsn_zero_rows(gsn_context* context, float* first, float* second, int stride, int row,
+    int rows);
+
+/* ---- density control ------------------------------------------------------------------- */
+
+/* Per-Gaussian decision {keep, clone, split, split-kept} and its exclusive prefix.
+ * Replaces CountDensityChangesKernel + DeviceScan::ExclusiveScan<int4> + the totals readback
+ * (density_controller.cu:60-116, 482-541). decisions/offsets are int4[N+1]; totals (host, 4
+ * ints) = offsets[N]. Synchronises the stream. */
+int gsn_density_count(gsn_context* context, const float* params, int count,
+    const float* gradient_sums, const float* absolute_gradient_sums,
+    const uint32_t* observation_counts, float gradient_threshold, float split_threshold,
+    float min_opacity, float max_world_size, int prune_large,
+    float absolute_gradient_threshold, int32_t* decisions, int32_t* offsets, int32_t* totals);
+
+/* Writes the compacted parameter buffer [keep | clone | split child 0 | split child 1].
+ * Split children move by R*(s (.) n), n ~ N(0, I) from Philox4x32-10 keyed by (seed, counter),
+ * and take log-scale log(s/1.6). Also writes history parent ids (kept -> old id, else -1),
+ * parent ids (old id for every output) and child flags (1 for clone/split outputs).
+ * Replaces GenerateDensitySamplesKernel, WriteDensityGaussiansKernel and
+ * FinishDensityControlKernel (density_controller.cu:138-278). Any of the three id/flag
+ * outputs may be NULL. */
+int gsn_density_write(gsn_context* context, const float* params, int count,
+    const int32_t* decisions, const int32_t* offsets, const int32_t* totals, uint64_t seed,
+    uint64_t counter, float* out_params, int out_count, int32_t* history_parent_ids,
+    int32_t* parent_ids, float* child_flags);
+
+/* The split-jitter generator on its own: out[i] = i-th standard normal for (seed, counter). */
+int gsn_density_normals(gsn_context* context, uint64_t seed, uint64_t counter, int64_t n,
+    float* out);
+
+/* sums += statistics[0], absolute_sums += absolute_statistics[0], counts += statistics[1]
+ * for ids < count. Replaces AccumulateRoundStatisticsKernel (density_controller.cu:280-295). */
+int gsn_density_accumulate(gsn_context* context, const float* statistics,
+    const float* absolute_statistics, int count, float* gradient_sums,
+    float* absolute_gradient_sums, uint32_t* observation_counts);
+
+/* Per-observation accumulation of |(gx, gy)| for radius > 0. gaussian_ids may be NULL
+ * (identity). Replaces AccumulateDensityStatisticsKernel (density_controller.cu:39-58). */
+int gsn_density_accumulate_observations(gsn_context* context, const float* gradients,
+    const float* radii, const int32_t* gaussian_ids, int observations, int count,
+    float* gradient_sums, uint32_t* observation_counts);
+
+/* logit <- min(logit, max_opacity_logit) on row 9 of a [58][stride] buffer, first count ids.
+ * Replaces ResetDensityOpacityKernel (density_controller.cu:297-308). */
+int gsn_density_reset_opacity(gsn_context* context, float* params, int stride, int count,
+    float max_opacity_logit);
+
+/* Zeroes the accumulators for the first count Gaussians. */
+int gsn_density_clear(gsn_context* context, float* gradient_sums,
+    float* absolute_gradient_sums, uint32_t* observation_counts, int count);
+
+/* ---- initialisation ---------------------------------------------------------------------- */
+
+/* Three nearest neighbours of every point (ties broken by smaller index) through a linear BVH
+ * over Morton-ordered points. neighbors[n][3] (optional) receives indices (-1 when fewer than
+ * 3 others exist); mean_dist2[n] receives the mean squared distance (reference rule for n < 4).
+ * points is device float[n][3]. Replaces ComputeMeanDist2Cuda's device part (knn.cu:180-255). */
+int gsn_knn3(gsn_context* context, const float* points, int n, int32_t* neighbors,
+    float* mean_dist2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSN_HIP_H */

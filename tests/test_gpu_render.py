@@ -1,0 +1,185 @@
+"""Renderer parity on the GPU: projection, tile lists and the forward rasteriser against the
+oracle (bit-exact integer outputs, FP32 mirror and FP64 reference images), plus the
+reference's own renderer cases (test/renderer_test.cu)."""
+import ctypes
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+from helpers import GpuRun, config0_camera, reference_camera
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
+
+
+def test_wave_sum12_lane_layout(gsn, context):
+    rng = np.random.default_rng(0)
+    values = rng.normal(size=(64, 12)).astype(np.float32)
+    source = gsn.DeviceBuffer.from_numpy(values)
+    out = gsn.DeviceBuffer((64, 3), np.float32)
+    gsn.call("gsn_selftest_wave_sum12", context.handle, source.ptr, out.ptr)
+    context.synchronize()
+    result = out.numpy()
+    totals = values.astype(np.float64).sum(axis=0)
+    for lane in range(64):
+        row = lane // 16
+        np.testing.assert_allclose(result[lane], totals[3 * row: 3 * row + 3], rtol=1e-5, atol=1e-5)
+
+
+def _compare_projection(run, expected):
+    actual = run.projected()
+    counts = expected["record_counts"][:-1]
+    np.testing.assert_array_equal(np.diff(run.projection.record_offsets.numpy()), counts)
+    visible = counts > 0
+    np.testing.assert_array_equal(actual["tile_rects"][visible], expected["tile_rects"][visible])
+    for key in ("means", "conic_opacities", "colors", "depths"):
+        a = actual[key][visible].view(np.uint32)
+        e = expected[key][visible].view(np.uint32)
+        mismatches = np.argwhere(a != e)
+        assert mismatches.size == 0, f"{key}: {len(mismatches)} bit mismatches, first {mismatches[:3]}"
+    return visible
+
+
+@pytest.mark.parametrize("sh_degree", [0, 1, 2, 3])
+def test_projection_and_tile_lists_bit_exact(gsn, context, oracle, scenes, sh_degree):
+    params = scenes.config0_params(seed=7 + sh_degree, count=10_000, sh_degree=3)
+    cam = config0_camera(gsn)
+    run = GpuRun(gsn, context, params, cam, sh_degree=sh_degree, keep_keys=True)
+    expected = oracle.project_f32(params, cam, sh_degree)
+    visible = _compare_projection(run, expected)
+    assert visible.sum() > 1000
+    lists = oracle.tile_lists(cam, expected)
+    actual = run.lists()
+    assert actual["record_count"] == lists["record_count"]
+    np.testing.assert_array_equal(actual["record_offsets"], lists["record_offsets"])
+    np.testing.assert_array_equal(actual["tile_offsets"], lists["tile_offsets"])
+    np.testing.assert_array_equal(actual["tile_gaussian_ids"], lists["tile_gaussian_ids"])
+    np.testing.assert_array_equal(actual["sorted_keys"], lists["sorted_keys"])
+
+
+def test_tile_lists_large_bit_exact(gsn, context, oracle, scenes):
+    """1080p, 200k Gaussians: several radix-sort tiles and a 13-bit tile key."""
+    params = scenes.blob_scene(200_000, seed=3, sh_degree=3)
+    params[3:6] = np.float32(-3.2)
+    R, t = scenes.look_at(np.array([0.3, -0.8, 4.5]))
+    f = scenes.fov_focal(1920, 60.0)
+    cam = gsn.Camera.make(1920, 1080, f, f, 959.5, 539.5, R, t)
+    run = GpuRun(gsn, context, params, cam, sh_degree=3, keep_keys=True)
+    expected = oracle.project_f32(params, cam, 3)
+    _compare_projection(run, expected)
+    lists = oracle.tile_lists(cam, expected)
+    actual = run.lists()
+    assert actual["record_count"] == lists["record_count"] > 100_000
+    np.testing.assert_array_equal(actual["tile_offsets"], lists["tile_offsets"])
+    np.testing.assert_array_equal(actual["tile_gaussian_ids"], lists["tile_gaussian_ids"])
+    np.testing.assert_array_equal(actual["sorted_keys"], lists["sorted_keys"])
+
+
+@pytest.mark.parametrize("sh_degree", [0, 3])
+def test_forward_matches_fp32_oracle(gsn, context, oracle, scenes, sh_degree):
+    params = scenes.config0_params(seed=11, count=10_000, sh_degree=3)
+    cam = config0_camera(gsn)
+    background = (0.05, 0.1, 0.15)
+    run = GpuRun(gsn, context, params, cam, background, sh_degree)
+    projected = oracle.project_f32(params, cam, sh_degree)
+    lists = oracle.tile_lists(cam, projected)
+    expected, ambiguous = oracle.render_f32(cam, projected, lists, background)
+    actual = run.image_host()
+    clear = ambiguous == 0
+    assert clear.mean() > 0.99, f"ambiguous pixels: {(~clear).sum()}"
+    error = np.abs(actual - expected)[clear]
+    tolerance = 1e-5 * np.abs(expected[clear]) + 1e-6
+    assert np.all(error <= tolerance), f"max error {error.max()}"
+
+
+def _reference_image(oracle, params, cam, background, degree):
+    return oracle.render_reference(params, cam, background, degree)
+
+
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+@pytest.mark.parametrize("size", [(9, 7), (19, 17), (33, 31)])
+def test_reference_case_sh_degrees_and_partial_tiles(gsn, context, oracle, scenes, degree, size):
+    """renderer_test.cu:9-29."""
+    params = scenes.make_gaussians(7)
+    cam = reference_camera(gsn, *size)
+    background = (0.05, 0.1, 0.15)
+    run = GpuRun(gsn, context, params, cam, background, degree)
+    expected = _reference_image(oracle, params, cam, background, degree)
+    actual = run.image_host().astype(np.float64)
+    assert np.all(np.isfinite(actual))
+    assert np.abs(actual - expected).max() < GOLDEN["render_linf"]
+
+
+def test_reference_case_empty_and_culled(gsn, context, scenes):
+    """renderer_test.cu:31-46: culled and empty scenes give the exact background."""
+    cam = reference_camera(gsn, 19, 17)
+    background = (0.2, 0.4, 0.6)
+    params = scenes.make_gaussians(2)
+    params[2, 0] = 0.1
+    params[2, 1] = 101.0
+    for count in (2, 0):
+        run = GpuRun(gsn, context, params[:, :count], cam, background, 0)
+        image = run.image_host()
+        np.testing.assert_array_equal(image, np.broadcast_to(np.float32(background), image.shape))
+
+
+def test_reference_case_depth_order_and_termination(gsn, context, oracle, scenes):
+    """renderer_test.cu:48-70."""
+    params = scenes.make_gaussians(8)
+    for i in range(8):
+        params[0:3, i] = [0.0, 0.0, 3.0 + i * 0.1]
+        params[3:6, i] = 0.0
+        params[9, i] = np.log(np.float32(49.0))
+        params[10, i] = (np.float32(0.2) + np.float32(i) * np.float32(0.08) - np.float32(0.5)) / np.float32(scenes.SH_C0)
+    params = params[:, ::-1].copy()
+    cam = reference_camera(gsn, 9, 7)
+    background = (0.1, 0.2, 0.3)
+    run = GpuRun(gsn, context, params, cam, background, 0)
+    expected = _reference_image(oracle, params, cam, background, 0)
+    assert np.abs(run.image_host() - expected).max() < GOLDEN["render_linf"]
+
+
+def test_reference_case_lower_colour_clamp(gsn, context, scenes):
+    """renderer_test.cu:72-88: clamp below at 0 only; the centre pixel reads 0 and 1.35."""
+    params = scenes.make_gaussians(1)
+    params[0:3, 0] = [0.0, 0.0, 3.0]
+    params[9, 0] = np.log(np.float32(9.0))
+    params[10, 0] = (np.float32(-0.2) - np.float32(0.5)) / np.float32(scenes.SH_C0)
+    params[26, 0] = (np.float32(1.5) - np.float32(0.5)) / np.float32(scenes.SH_C0)
+    cam = reference_camera(gsn, 9, 7)
+    run = GpuRun(gsn, context, params, cam, (0.0, 0.0, 0.0), 0)
+    pixel = run.image_host()[3, 4]
+    clamp = GOLDEN["color_clamp"]
+    assert pixel[0] == clamp["red"]
+    assert abs(pixel[1] - clamp["green"]) < clamp["tolerance"]
+
+
+@pytest.mark.parametrize("count", [257, 513])
+def test_reference_case_multiple_batches(gsn, context, oracle, scenes, count):
+    """renderer_test.cu:89-114: tile lists longer than one batch."""
+    params = scenes.make_gaussians(count)
+    for i in range(count):
+        params[0:3, i] = [0.1, -0.1, 3.0 + 0.001 * i]
+        params[3:6, i] = np.log(np.float32(0.8))
+        params[9, i] = -4.4
+        params[10, i] += np.float32(0.1) * np.sin(np.float32(i))
+    params = params[:, ::-1].copy()
+    cam = reference_camera(gsn, 9, 7)
+    background = (0.1, 0.2, 0.3)
+    run = GpuRun(gsn, context, params, cam, background, 0)
+    expected = _reference_image(oracle, params, cam, background, 0)
+    assert np.abs(run.image_host() - expected).max() < GOLDEN["render_linf"]
+
+
+def test_image_metrics_match_oracle_loss(gsn, context, oracle, scenes):
+    rng = np.random.default_rng(5)
+    image = rng.uniform(0, 1, size=(70, 45, 3)).astype(np.float32)
+    target = rng.integers(0, 256, size=(70, 45, 3), dtype=np.uint8)
+    metrics = gsn.image_metrics(context, 45, 70, gsn.DeviceBuffer.from_numpy(image), gsn.DeviceBuffer.from_numpy(target))
+    expected = oracle.loss_f64(image.astype(np.float64), target)
+    assert abs(metrics["loss"] - expected) < 1e-5 * max(1.0, abs(expected))
+    mse = np.mean((np.clip(image, 0, 1).astype(np.float64) - target / 255.0) ** 2)
+    assert abs(metrics["mse"] - mse) < 1e-6
