@@ -118,6 +118,10 @@ _PROTOTYPES = {
     "gsn_density_reset_opacity": [_P, _P, _I, _I, _F],
     "gsn_density_clear": [_P, _P, _P, _P, _I],
     "gsn_knn3": [_P, _P, _I, _P, _P],
+    "gsn_mlp_sizes": [ctypes.POINTER(_I), ctypes.POINTER(_I)],
+    "gsn_mlp_sync_weights": [_P, _P, _P],
+    "gsn_mlp_forward": [_P, _P, _I, _P, _P, _P],
+    "gsn_mlp_train_step": [_P, _P, _P, _P, _I, _P, _P, _P, _P, _F, _I, _P, ctypes.POINTER(ctypes.c_double)],
     "gsn_selftest_wave_sum12": [_P, _P, _P],
     "gsn_context_set_timing": [_P, _I],
     "gsn_context_stage_times": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), _I],
@@ -404,3 +408,37 @@ def knn3(context: Context, points: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     call("gsn_knn3", context.handle, _p(device_points), n, _p(neighbors), _p(distances))
     context.synchronize()
     return neighbors.numpy(), distances.numpy()
+
+
+class ProposalNetwork:
+    """Device state of the 16 -> 64 -> 64 -> 3 densification proposal network: FP32 master
+    parameters, Adam moments and the BF16 copy read by the MFMA kernels."""
+
+    def __init__(self, context: Context, params: np.ndarray):
+        sizes = (ctypes.c_int(), ctypes.c_int())
+        call("gsn_mlp_sizes", ctypes.byref(sizes[0]), ctypes.byref(sizes[1]))
+        self.parameter_count, self.weight_count = sizes[0].value, sizes[1].value
+        params = np.ascontiguousarray(params, np.float32)
+        if params.shape != (self.parameter_count,):
+            raise ValueError(f"expected {self.parameter_count} parameters, got {params.shape}")
+        self.context = context
+        self.params = DeviceBuffer.from_numpy(params)
+        self.first = DeviceBuffer.from_numpy(np.zeros_like(params))
+        self.second = DeviceBuffer.from_numpy(np.zeros_like(params))
+        self.weights = DeviceBuffer((self.weight_count,), np.uint16)
+        self.step = 0
+        call("gsn_mlp_sync_weights", context.handle, _p(self.params), _p(self.weights))
+
+    def forward(self, features: DeviceBuffer, count: int, logits: DeviceBuffer) -> None:
+        call("gsn_mlp_forward", self.context.handle, _p(features), count, _p(self.weights), _p(self.params),
+             _p(logits))
+
+    def train_step(self, features: DeviceBuffer, labels: DeviceBuffer, count: int, learning_rate: float,
+                   sample_weights: DeviceBuffer | None = None, gradients: DeviceBuffer | None = None,
+                   want_loss: bool = False) -> float | None:
+        self.step += 1
+        loss = ctypes.c_double(0.0)
+        call("gsn_mlp_train_step", self.context.handle, _p(features), _p(labels), _p(sample_weights), count,
+             _p(self.params), _p(self.first), _p(self.second), _p(self.weights), learning_rate, self.step,
+             _p(gradients), ctypes.byref(loss) if want_loss else None)
+        return loss.value if want_loss else None

@@ -161,6 +161,9 @@ __global__ __launch_bounds__(256) void density_write_kernel(const float* __restr
     for (int child = 0; child < children; ++child) {
         int destination = split ? totals.x + totals.y + child * totals.w + o.w
                                 : (child == 0 && d.x ? o.x : totals.x + o.y);
+        if (destination < 0 || destination >= out_count) {
+            continue;  // inconsistent decisions/offsets from the caller: never write out of bounds
+        }
         float normals[3] = {0.0f, 0.0f, 0.0f};
         if (split) {
 #pragma unroll

@@ -299,7 +299,7 @@ struct AdamSink {
 
 template<int Degree, bool Fused>
 __global__ __launch_bounds__(256) void gaussian_gradients_kernel(const float* __restrict__ partials,
-    const int* __restrict__ record_offsets, int count, Camera cam, float* params, float* gradients, float* screen,
+    const int* __restrict__ record_offsets, int count, int record_count, Camera cam, float* params, float* gradients, float* screen,
     float* statistics, float* absolute_statistics, float* first, float* second, float position_learning_rate,
     float first_correction, float second_correction) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void gaussian_gradients_kernel(const float* __
     }
     const int64_t n = count;
     const int begin = record_offsets[id];
-    const int end = record_offsets[id + 1];
+    const int end = min(record_offsets[id + 1], record_count);
     float in[11];
 #pragma unroll
     for (int k = 0; k < 11; ++k) {
@@ -423,27 +423,27 @@ __global__ __launch_bounds__(256) void zero_rows_kernel(float* first, float* sec
 }
 
 template<bool Fused>
-int launch_gradients(gsn_context* context, const Camera& cam, int count, int sh_degree, const float* partials,
-    const int* record_offsets, float* params, float* gradients, float* screen, float* statistics,
+int launch_gradients(gsn_context* context, const Camera& cam, int count, int record_count, int sh_degree,
+    const float* partials, const int* record_offsets, float* params, float* gradients, float* screen, float* statistics,
     float* absolute_statistics, float* first, float* second, float lr, float c1, float c2) {
     unsigned grid = grid_for(count, 256);
     hipStream_t stream = context->stream;
     StageScope scope(context, kStageGaussianGradients);
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
-        gaussian_gradients_kernel<0, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, cam, params,
+        gaussian_gradients_kernel<0, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     case 1:
-        gaussian_gradients_kernel<1, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, cam, params,
+        gaussian_gradients_kernel<1, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     case 2:
-        gaussian_gradients_kernel<2, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, cam, params,
+        gaussian_gradients_kernel<2, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     default:
-        gaussian_gradients_kernel<3, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, cam, params,
+        gaussian_gradients_kernel<3, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     }
@@ -494,9 +494,9 @@ extern "C" int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* ca
     }
     if (record_count > 0) {
         GSN_TRY(launch_splat_backward(context, cam, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors,
-            tile_rects, record_offsets, image, image_gradients, partials));
+            tile_rects, record_offsets, image, image_gradients, partials, record_count));
     }
-    return launch_gradients<false>(context, cam, count, sh_degree, partials, record_offsets, const_cast<float*>(params),
+    return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets, const_cast<float*>(params),
         gradients, screen_gradients, statistics, absolute_statistics, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
 }
 
@@ -525,9 +525,9 @@ extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camer
     }
     if (record_count > 0) {
         GSN_TRY(launch_splat_backward(context, cam, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors,
-            tile_rects, record_offsets, image, image_gradients, partials));
+            tile_rects, record_offsets, image, image_gradients, partials, record_count));
     }
-    return launch_gradients<true>(context, cam, count, sh_degree, partials, record_offsets, params, nullptr, nullptr,
+    return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, params, nullptr, nullptr,
         statistics, absolute_statistics, first_moments, second_moments, position_learning_rate, first_correction,
         second_correction);
 }

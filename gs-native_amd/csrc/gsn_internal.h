@@ -113,7 +113,7 @@ constexpr int kPartialFloats = 12;
 // Tile rasteriser backward writing per-record partials (raster_backward.hip).
 int launch_splat_backward(gsn_context* context, const Camera& cam, const int* ids, const int64_t* tile_offsets,
     const float* means, const float* conics, const float* colors, const int* rects, const int* record_offsets,
-    const float* image, const float* image_gradients, float* partials);
+    const float* image, const float* image_gradients, float* partials, int record_count);
 
 inline unsigned grid_for(int64_t n, int block) {
     return static_cast<unsigned>((n + block - 1) / block);

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
     const int64_t* __restrict__ tile_offsets, const float2* __restrict__ means, const float4* __restrict__ conics,
     const float* __restrict__ colors, const int4* __restrict__ rects, const int* __restrict__ record_offsets,
     Camera cam, const float* __restrict__ image, const float* __restrict__ image_gradients, float* __restrict__ partials,
-    int tile_count) {
+    int tile_count, int record_count) {
     __shared__ float s_partial[64 * kPartialFloats];
     const int tile = xcd_remap(blockIdx.x, tile_count);
     const int lane = threadIdx.x;
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
             }
         }
         __syncthreads();
-        if (valid) {
+        if (valid && slot >= 0 && slot < record_count) {
             const float4* source = reinterpret_cast<const float4*>(s_partial + lane * kPartialFloats);
             float4* destination = reinterpret_cast<float4*>(partials + slot * kPartialFloats);
             destination[0] = source[0];
@@ -187,6 +187,9 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
         if (record < end) {
             int id = ids[record];
             int64_t slot = partial_slot(id, tile_x, tile_y, rects, record_offsets);
+            if (slot < 0 || slot >= record_count) {
+                continue;
+            }
             float4* destination = reinterpret_cast<float4*>(partials + slot * kPartialFloats);
             float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             destination[0] = zero;
@@ -215,13 +218,13 @@ __global__ __launch_bounds__(64) void wave_sum12_selftest_kernel(const float* in
 
 int launch_splat_backward(gsn_context* context, const Camera& cam, const int* ids, const int64_t* tile_offsets,
     const float* means, const float* conics, const float* colors, const int* rects, const int* record_offsets,
-    const float* image, const float* image_gradients, float* partials) {
+    const float* image, const float* image_gradients, float* partials, int record_count) {
     int64_t tiles = static_cast<int64_t>(cam.tile_count_x) * cam.tile_count_y;
     StageScope scope(context, kStageRasterBackward);
     raster_backward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets,
         reinterpret_cast<const float2*>(means), reinterpret_cast<const float4*>(conics), colors,
         reinterpret_cast<const int4*>(rects), record_offsets, cam, image, image_gradients, partials,
-        static_cast<int>(tiles));
+        static_cast<int>(tiles), record_count);
     return check_launch("raster_backward_kernel");
 }
 

@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void depth_order_counts_kernel(const uint32_t*
 constexpr int kEmitLanes = 8;
 
 __global__ __launch_bounds__(256) void emit_records_kernel(const uint32_t* sorted_ids, const int* depth_offsets,
-    const int4* tile_rects, uint32_t* record_tiles, uint32_t* record_ids, int count, int tile_count_x) {
+    const int4* tile_rects, uint32_t* record_tiles, uint32_t* record_ids, int count, int tile_count_x,
+    int record_count) {
     int k = (blockIdx.x * blockDim.x + threadIdx.x) / kEmitLanes;
     int lane = threadIdx.x % kEmitLanes;
     if (k >= count) {
@@ -57,7 +58,8 @@ __global__ __launch_bounds__(256) void emit_records_kernel(const uint32_t* sorte
     uint32_t id = sorted_ids[k];
     int4 rect = tile_rects[id];
     int width = rect.z - rect.x;
-    for (int local = lane; local < total; local += kEmitLanes) {
+    // record_count bounds the writes even if record_offsets disagree with it.
+    for (int local = lane; local < total && begin + local < record_count; local += kEmitLanes) {
         int row = local / width;
         int column = local - row * width;
         record_tiles[begin + local] = static_cast<uint32_t>((rect.y + row) * tile_count_x + rect.x + column);
@@ -71,8 +73,10 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* sorted
     if (r >= records) {
         return;
     }
-    int tile = static_cast<int>(sorted_tiles[r]);
-    int previous = r == 0 ? -1 : static_cast<int>(sorted_tiles[r - 1]);
+    // Tile ids come from rectangles clamped to the grid; min() keeps writes inside offsets[0..T]
+    // even for rectangles produced with a different camera.
+    int tile = min(static_cast<int>(sorted_tiles[r]), tile_count);
+    int previous = r == 0 ? -1 : min(static_cast<int>(sorted_tiles[r - 1]), tile_count);
     if (tile != previous) {
         for (int t = previous + 1; t <= tile; ++t) {
             offsets[t] = r;
@@ -169,7 +173,7 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
         GSN_TRY(exclusive_scan_int(context, depth_counts, depth_counts, static_cast<int64_t>(count) + 1));
         emit_records_kernel<<<grid_for(static_cast<int64_t>(count) * kEmitLanes, 256), 256, 0, stream>>>(sorted_ids,
             depth_counts, reinterpret_cast<const int4*>(tile_rects), record_tiles, record_ids, count,
-            camera->tile_count_x);
+            camera->tile_count_x, record_count);
         GSN_TRY(check_launch("emit_records_kernel"));
     }
     int tile_bits = 0;

@@ -45,10 +45,14 @@ bool DensityController::Reserve(int capacity) {
             Check(gsn_context_synchronize(context), "gsn_context_synchronize");
     }
     if (ok && capacity_ > 0) {
+        // Decisions and offsets are copied too: ControlDensity grows the buffers between the
+        // count pass that fills them and the write pass that reads them.
         size_t old = static_cast<size_t>(capacity_);
         ok = hipMemcpy(sums, gradient_sums_, old * sizeof(float), hipMemcpyDeviceToDevice) == hipSuccess &&
             hipMemcpy(absolute, absolute_gradient_sums_, old * sizeof(float), hipMemcpyDeviceToDevice) == hipSuccess &&
-            hipMemcpy(counts, observation_counts_, old * sizeof(uint32_t), hipMemcpyDeviceToDevice) == hipSuccess;
+            hipMemcpy(counts, observation_counts_, old * sizeof(uint32_t), hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(decisions, density_counts_, 4 * (old + 1) * sizeof(int), hipMemcpyDeviceToDevice) == hipSuccess &&
+            hipMemcpy(offsets, density_offsets_, 4 * (old + 1) * sizeof(int), hipMemcpyDeviceToDevice) == hipSuccess;
     }
     if (!ok) {
         DeviceFree(sums);

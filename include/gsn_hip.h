@@ -247,6 +247,33 @@ int gsn_density_clear(gsn_context* context, float* gradient_sums,
 int gsn_knn3(gsn_context* context, const float* points, int n, int32_t* neighbors,
     float* mean_dist2);
 
+/* ---- learned densification proposal (no counterpart in the reference) ------------------ */
+
+/* Number of FP32 master parameters (5443: W1[64][16], b1[64], W2[64][64], b2[64], W3[3][64],
+ * b3[3]) and of BF16 weight entries (7168: W1[64][16], W2[64][64], W3 padded to [32][64]) of
+ * the 16 -> 64 -> 64 -> 3 proposal network. */
+int gsn_mlp_sizes(int* parameters, int* bf16_weights);
+
+/* Rounds the FP32 master weights to the BF16 copy the MFMA kernels read (round to nearest
+ * even), zeroing the W3 padding rows. weights is uint16_t[7168] holding BF16 bit patterns. */
+int gsn_mlp_sync_weights(gsn_context* context, const float* params, uint16_t* weights);
+
+/* logits[n][3] = W3 relu(W2 relu(W1 x + b1) + b2) + b3 for features[n][16] (FP32 in HBM,
+ * rounded to BF16 for the MFMA operands; hidden activations are rounded to BF16 after the
+ * FP32 bias + ReLU; accumulation is FP32). Biases come from the FP32 parameter block. */
+int gsn_mlp_forward(gsn_context* context, const float* features, int count,
+    const uint16_t* weights, const float* params, float* logits);
+
+/* One full-batch training step on the weighted mean softmax cross-entropy over labels
+ * (0 keep, 1 clone, 2 split): gradients of all 5443 parameters (MFMA forward and backward,
+ * per-workgroup sums reduced in a fixed order), then Adam (beta 0.9 / 0.999, eps 1e-8, bias
+ * corrections for `step`) on params/first/second when learning_rate > 0, and a refresh of the
+ * BF16 copy. sample_weights may be NULL (all 1). gradients_out[5443] and loss_out (host; the
+ * loss before the update) are optional; loss_out synchronises the stream. */
+int gsn_mlp_train_step(gsn_context* context, const float* features, const int32_t* labels,
+    const float* sample_weights, int count, float* params, float* first, float* second,
+    uint16_t* weights, float learning_rate, int step, float* gradients_out, double* loss_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -946,27 +946,36 @@ void oracle_backward_f64(const oracle_camera* cam, int count, const double* mean
                 const double* col = colors + 3 * id;
                 double weight = alpha * T;
                 double alpha_gradient = 0.0;
+                /* Error-bound magnitude of alpha_gradient: the sum of the absolute values of the
+                 * summands prefix + T c - C, which cancel when T is small. An FP32 evaluation of
+                 * the same expression is accurate relative to this, not to |alpha_gradient|. */
+                double alpha_magnitude = 0.0;
                 double term;
                 for (int c = 0; c < 3; ++c) {
                     alpha_gradient += g[c] * (prefix[c] + T * col[c] - final_color[c]) / (1.0 - alpha);
+                    alpha_magnitude +=
+                        fabs(g[c]) * (fabs(prefix[c]) + fabs(T * col[c]) + fabs(final_color[c])) / (1.0 - alpha);
                     term = weight * g[c];
                     screen[(6 + c) * n + id] += term;
                     magnitude[(6 + c) * n + id] += fabs(term);
                 }
                 if (raw_alpha < 0.99) {
                     double common = alpha_gradient * raw_alpha;
+                    double common_magnitude = alpha_magnitude * raw_alpha;
                     double qx = q[0] * dx + q[1] * dy;
                     double qy = q[1] * dx + q[2] * dy;
                     double terms[6] = {-common * qx, -common * qy, 0.5 * common * qx * qx, common * qx * qy,
                         0.5 * common * qy * qy, common * (1.0 - q[3])};
+                    double bounds[6] = {fabs(qx), fabs(qy), 0.5 * qx * qx, fabs(qx * qy), 0.5 * qy * qy,
+                        fabs(1.0 - q[3])};
                     for (int k = 0; k < 6; ++k) {
                         screen[k * n + id] += terms[k];
-                        magnitude[k * n + id] += fabs(terms[k]);
+                        magnitude[k * n + id] += common_magnitude * bounds[k];
                     }
                     screen[9 * n + id] += fabs(terms[0]);
                     screen[10 * n + id] += fabs(terms[1]);
-                    magnitude[9 * n + id] += fabs(terms[0]);
-                    magnitude[10 * n + id] += fabs(terms[1]);
+                    magnitude[9 * n + id] += common_magnitude * bounds[0];
+                    magnitude[10 * n + id] += common_magnitude * bounds[1];
                 }
                 for (int c = 0; c < 3; ++c) {
                     prefix[c] += col[c] * weight;

@@ -77,7 +77,9 @@ void oracle_image_gradients_f64(const double* image, const uint8_t* target, int 
 
 /* solver.cu:325-466 in FP64: per-pixel front-to-back replay from the final colour. screen[11][N]
  * receives dmean2D(2), dSigma(3), dlogit, dcolor(3), |dmean2D|(2); magnitude[11][N] the sums of
- * |term| (scale for tolerances); tainted[N] marks Gaussians that touch an ambiguous pixel. */
+ * the per-pixel terms' error-bound magnitudes (for the geometry fields, dL/dalpha's summands
+ * |g| (|prefix| + |T c| + |C|) / (1 - alpha) in place of the cancelled |dL/dalpha|), the scale
+ * for tolerances; tainted[N] marks Gaussians that touch an ambiguous pixel. */
 void oracle_backward_f64(const oracle_camera* camera, int count, const double* means, const double* conics,
     const double* colors, const double* depths, const int* tile_rects, const uint8_t* visible, const double* image,
     const double* image_gradients, double* screen, double* magnitude, uint8_t* tainted);

@@ -2,7 +2,11 @@
 history remap against the oracle, plus the reference's own solver cases (test/solver_test.cu).
 
 Tolerance rule for gradients (sums over many pixels cancel): |gpu - oracle| <= 1e-5 * sum|terms|
-+ 1e-9 * max|row|, where sum|terms| is the oracle's sum of absolute per-pixel contributions.
++ 1e-9 * max|row|, where sum|terms| is the oracle's sum of the per-pixel terms' error-bound
+magnitudes. For the geometry fields that magnitude is built from |g| (|prefix| + |T c| + |C|) /
+(1 - alpha), the summands of dL/dalpha before they cancel: the replay from the final colour
+(solver.cu:389-396) subtracts nearly equal FP32 colours behind the last splats of a pixel, so its
+error is relative to those summands, as it is in the reference's own FP32 kernel.
 """
 import json
 import pathlib
