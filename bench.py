@@ -12,6 +12,7 @@ from __future__ import annotations
 import argparse
 import importlib.util
 import json
+import re
 import math
 import os
 import pathlib
@@ -56,6 +57,17 @@ def algorithmic_bytes(n: int, v: float, r: float, p: int, t: int, k: int = 16) -
         "raster_backward": 40 * r + 24 * p + 44 * v,
         "gaussian_gradients": 32 * n + 500 * v + 1624 * n,
     }
+
+
+def measured_traffic(stage: str):
+    """HBM bytes per launch of a stage's kernel as measured by rocprofv3 FETCH_SIZE + WRITE_SIZE on
+    this workload (committed under profiles/, scaled by a plain copy of known size), or None."""
+    for path in sorted((REPO / "profiles").glob("*.md")):
+        for line in path.read_text().splitlines():
+            match = re.match(r"^traffic_bytes_per_launch\s+(\w+)\s*=\s*([0-9.eE+]+)", line.strip())
+            if match and match.group(1) == stage:
+                return float(match.group(2))
+    return None
 
 
 def build_workload(gsn, context, gaussians, width, height, cameras, seed=0):
@@ -143,10 +155,13 @@ def main() -> int:
     parser.add_argument("--start-iteration", type=int, default=3000)
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--calibration-copy-mib", type=int, default=0,
+                        help="after the timed steps, run one device-to-device copy of this many MiB (a plain copy "
+                             "kernel of known size for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = parser.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build runs on a single "
-              f"MI355X; see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the training step runs on a "
+              f"single MI355X; multi-GPU sharding is not implemented).", file=sys.stderr)
         return 2
     gsn = load_package()
     trainer_spec = importlib.util.spec_from_file_location("gs_native_amd.trainer", REPO / "gs-native_amd" / "trainer.py")
@@ -234,6 +249,15 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    result["roofline"]["traffic"] = measured_traffic(dominant)
+    if args.calibration_copy_mib > 0:
+        size = args.calibration_copy_mib << 20
+        source = gsn.DeviceBuffer((size,), np.uint8)
+        destination = gsn.DeviceBuffer((size,), np.uint8)
+        gsn.call("gsn_memcpy_device", destination.ptr, source.ptr, size)
+        context.synchronize()
+        log(f"calibration copy of {size} bytes done")
+        del source, destination
     if not args.no_cpu_baseline:
         log("timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

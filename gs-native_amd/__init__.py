@@ -118,6 +118,12 @@ _PROTOTYPES = {
     "gsn_density_reset_opacity": [_P, _P, _I, _I, _F],
     "gsn_density_clear": [_P, _P, _P, _P, _I],
     "gsn_knn3": [_P, _P, _I, _P, _P],
+    "gsn_density_count_proposed": [_P, _P, _I, _P, _P, _P, _P, _F, _F, _F, _I, _P, _P,
+                                   ctypes.POINTER(ctypes.c_int32)],
+    "gsn_density_features_clear": [_P, _P, _I],
+    "gsn_density_features_accumulate": [_P, _P, _P, _P, _P, _I, _P, _I],
+    "gsn_density_features": [_P, _P, _I, _P, _P, _P, _P, _I, _F, _F, _F, _P],
+    "gsn_density_outcomes": [_P, _P, _P, ctypes.POINTER(ctypes.c_int32), _I, _P, _P, _I, _F, _P],
     "gsn_mlp_sizes": [ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "gsn_mlp_sync_weights": [_P, _P, _P],
     "gsn_mlp_forward": [_P, _P, _I, _P, _P, _P],

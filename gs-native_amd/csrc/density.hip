@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void density_count_kernel(const float* __restr
     const float* __restrict__ gradient_sums, const float* __restrict__ absolute_gradient_sums,
     const uint32_t* __restrict__ observation_counts, Int4* decisions, int count, float gradient_threshold,
     float split_threshold, float min_opacity, float max_world_size, int prune_large,
-    float absolute_gradient_threshold) {
+    float absolute_gradient_threshold, const float* __restrict__ proposal) {
 #pragma clang fp contract(off)
     int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id > count) {
@@ -88,12 +88,25 @@ __global__ __launch_bounds__(256) void density_count_kernel(const float* __restr
     if (scale[0] != scale[0] || scale[1] != scale[1] || scale[2] != scale[2]) {
         max_scale = bits_to_float(0x7fc00000u);
     }
-    bool selected = gradient >= gradient_threshold;
-    bool clone = selected && max_scale <= split_threshold;
-    float absolute_gradient = observations ? absolute_gradient_sums[id] / static_cast<float>(observations) : 0.0f;
-    bool split_selected =
-        absolute_gradient_threshold > 0.0f ? absolute_gradient >= absolute_gradient_threshold : selected;
-    bool split = split_selected && max_scale > split_threshold;
+    bool clone, split;
+    if (proposal) {
+        // Learned proposal: the class with the largest logit (ties and NaN favour keep, then
+        // clone) replaces the gradient-threshold selection; pruning below stays rule-based.
+        float keep_logit = proposal[3 * static_cast<int64_t>(id)];
+        float clone_logit = proposal[3 * static_cast<int64_t>(id) + 1];
+        float split_logit = proposal[3 * static_cast<int64_t>(id) + 2];
+        clone = clone_logit > keep_logit && clone_logit >= split_logit;
+        split = split_logit > keep_logit && split_logit > clone_logit;
+    }
+    else {
+        bool selected = gradient >= gradient_threshold;
+        clone = selected && max_scale <= split_threshold;
+        float absolute_gradient =
+            observations ? absolute_gradient_sums[id] / static_cast<float>(observations) : 0.0f;
+        bool split_selected =
+            absolute_gradient_threshold > 0.0f ? absolute_gradient >= absolute_gradient_threshold : selected;
+        split = split_selected && max_scale > split_threshold;
+    }
     float remaining = max_scale;
     if (split && prune_large) {
         float x = det_expf(det_logf(scale[0] / 1.6f));
@@ -256,11 +269,14 @@ __global__ __launch_bounds__(256) void density_clear_kernel(float* sums, float* 
 } // namespace
 } // namespace gsn
 
-extern "C" int gsn_density_count(gsn_context* context, const float* params, int count, const float* gradient_sums,
+namespace gsn {
+namespace {
+
+int density_count(gsn_context* context, const float* params, int count, const float* gradient_sums,
     const float* absolute_gradient_sums, const uint32_t* observation_counts, float gradient_threshold,
     float split_threshold, float min_opacity, float max_world_size, int prune_large,
-    float absolute_gradient_threshold, int32_t* decisions, int32_t* offsets, int32_t* totals) {
-    using namespace gsn;
+    float absolute_gradient_threshold, const float* proposal, int32_t* decisions, int32_t* offsets,
+    int32_t* totals) {
     if (!context || count < 0 || !decisions || !offsets || !totals) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_density_count: invalid arguments");
     }
@@ -272,7 +288,7 @@ extern "C" int gsn_density_count(gsn_context* context, const float* params, int 
     StageScope scope(context, kStageDensity);
     density_count_kernel<<<grid_for(static_cast<int64_t>(count) + 1, 256), 256, 0, context->stream>>>(params,
         gradient_sums, absolute_gradient_sums, observation_counts, d, count, gradient_threshold, split_threshold,
-        min_opacity, max_world_size, prune_large, absolute_gradient_threshold);
+        min_opacity, max_world_size, prune_large, absolute_gradient_threshold, proposal);
     GSN_TRY(check_launch("density_count_kernel"));
     GSN_TRY(exclusive_scan_int4(context, d, o, static_cast<int64_t>(count) + 1));
     int* pinned = static_cast<int*>(context->pinned);
@@ -282,6 +298,30 @@ extern "C" int gsn_density_count(gsn_context* context, const float* params, int 
         totals[k] = pinned[k];
     }
     return GSN_OK;
+}
+
+} // namespace
+} // namespace gsn
+
+extern "C" int gsn_density_count(gsn_context* context, const float* params, int count, const float* gradient_sums,
+    const float* absolute_gradient_sums, const uint32_t* observation_counts, float gradient_threshold,
+    float split_threshold, float min_opacity, float max_world_size, int prune_large,
+    float absolute_gradient_threshold, int32_t* decisions, int32_t* offsets, int32_t* totals) {
+    return gsn::density_count(context, params, count, gradient_sums, absolute_gradient_sums, observation_counts,
+        gradient_threshold, split_threshold, min_opacity, max_world_size, prune_large, absolute_gradient_threshold,
+        nullptr, decisions, offsets, totals);
+}
+
+extern "C" int gsn_density_count_proposed(gsn_context* context, const float* params, int count,
+    const float* gradient_sums, const float* absolute_gradient_sums, const uint32_t* observation_counts,
+    const float* proposal_logits, float split_threshold, float min_opacity, float max_world_size, int prune_large,
+    int32_t* decisions, int32_t* offsets, int32_t* totals) {
+    if (count > 0 && !proposal_logits) {
+        return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_density_count_proposed: null proposal logits");
+    }
+    return gsn::density_count(context, params, count, gradient_sums, absolute_gradient_sums, observation_counts,
+        0.0f, split_threshold, min_opacity, max_world_size, prune_large, 0.0f, proposal_logits, decisions, offsets,
+        totals);
 }
 
 extern "C" int gsn_density_write(gsn_context* context, const float* params, int count, const int32_t* decisions,

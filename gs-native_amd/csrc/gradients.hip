@@ -297,37 +297,59 @@ struct AdamSink {
     }
 };
 
+constexpr int kGradientBlock = 256;
+constexpr int kGradientChunk = 256;  // partial records staged per LDS round (12 KiB)
+
 template<int Degree, bool Fused>
-__global__ __launch_bounds__(256) void gaussian_gradients_kernel(const float* __restrict__ partials,
+__global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const float* __restrict__ partials,
     const int* __restrict__ record_offsets, int count, int record_count, Camera cam, float* params, float* gradients, float* screen,
     float* statistics, float* absolute_statistics, float* first, float* second, float position_learning_rate,
     float first_correction, float second_correction) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= count) {
-        return;
-    }
+    // The block's Gaussians own one contiguous run of partial records. It is staged through LDS
+    // in chunks with coalesced 16-byte loads; each thread then sums its own records in order.
+    __shared__ float4 s_records[kGradientChunk * 3];
+    const int first_id = blockIdx.x * kGradientBlock;
+    const int id = first_id + threadIdx.x;
     const int64_t n = count;
-    const int begin = record_offsets[id];
-    const int end = min(record_offsets[id + 1], record_count);
+    const int block_begin = min(record_offsets[first_id], record_count);
+    const int block_end = min(record_offsets[min(first_id + kGradientBlock, count)], record_count);
+    int begin = 0, end = 0;
+    if (id < count) {
+        begin = min(record_offsets[id], record_count);
+        end = min(record_offsets[id + 1], record_count);
+    }
     float in[11];
 #pragma unroll
     for (int k = 0; k < 11; ++k) {
         in[k] = 0.0f;
     }
-    for (int p = begin; p < end; ++p) {
-        const float4* record = reinterpret_cast<const float4*>(partials + static_cast<int64_t>(p) * kPartialFloats);
-        float4 a = record[0], b = record[1], c = record[2];
-        in[0] += a.x;
-        in[1] += a.y;
-        in[2] += a.z;
-        in[3] += a.w;
-        in[4] += b.x;
-        in[5] += b.y;
-        in[6] += b.z;
-        in[7] += b.w;
-        in[8] += c.x;
-        in[9] += c.y;
-        in[10] += c.z;
+    const float4* records = reinterpret_cast<const float4*>(partials);
+    for (int chunk = block_begin; chunk < block_end; chunk += kGradientChunk) {
+        const int chunk_end = min(chunk + kGradientChunk, block_end);
+        for (int i = threadIdx.x; i < (chunk_end - chunk) * 3; i += kGradientBlock) {
+            s_records[i] = records[static_cast<int64_t>(chunk) * 3 + i];
+        }
+        __syncthreads();
+        const int last = min(end, chunk_end);
+        for (int p = max(begin, chunk); p < last; ++p) {
+            const float4 a = s_records[(p - chunk) * 3], b = s_records[(p - chunk) * 3 + 1],
+                         c = s_records[(p - chunk) * 3 + 2];
+            in[0] += a.x;
+            in[1] += a.y;
+            in[2] += a.z;
+            in[3] += a.w;
+            in[4] += b.x;
+            in[5] += b.y;
+            in[6] += b.z;
+            in[7] += b.w;
+            in[8] += c.x;
+            in[9] += c.y;
+            in[10] += c.z;
+        }
+        __syncthreads();
+    }
+    if (id >= count) {
+        return;
     }
     {
         float x = in[0] * (0.5f * static_cast<float>(cam.width));
@@ -426,24 +448,24 @@ template<bool Fused>
 int launch_gradients(gsn_context* context, const Camera& cam, int count, int record_count, int sh_degree,
     const float* partials, const int* record_offsets, float* params, float* gradients, float* screen, float* statistics,
     float* absolute_statistics, float* first, float* second, float lr, float c1, float c2) {
-    unsigned grid = grid_for(count, 256);
+    unsigned grid = grid_for(count, kGradientBlock);
     hipStream_t stream = context->stream;
     StageScope scope(context, kStageGaussianGradients);
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
-        gaussian_gradients_kernel<0, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
+        gaussian_gradients_kernel<0, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     case 1:
-        gaussian_gradients_kernel<1, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
+        gaussian_gradients_kernel<1, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     case 2:
-        gaussian_gradients_kernel<2, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
+        gaussian_gradients_kernel<2, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     default:
-        gaussian_gradients_kernel<3, Fused><<<grid, 256, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
+        gaussian_gradients_kernel<3, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     }

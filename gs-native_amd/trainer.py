@@ -27,10 +27,44 @@ else:  # loaded by path
                                                              _pkg.render, _pkg._p)
 
 
+FEATURES = 16
+FEATURE_STATE_ROWS = 6
+
+
+class OutcomeLog:
+    """Training samples for the densification proposal: the features of every Gaussian at a
+    density step paired with the outcome label of that step (gsn_density_outcomes), collected on
+    the host. Pruned Gaussians (label -1) are dropped."""
+
+    def __init__(self):
+        self.features: list[np.ndarray] = []
+        self.labels: list[np.ndarray] = []
+
+    def add(self, features: np.ndarray, labels: np.ndarray) -> None:
+        keep = labels >= 0
+        self.features.append(features[keep])
+        self.labels.append(labels[keep].astype(np.int32))
+
+    def arrays(self) -> tuple[np.ndarray, np.ndarray]:
+        if not self.features:
+            return np.zeros((0, FEATURES), np.float32), np.zeros(0, np.int32)
+        return np.concatenate(self.features), np.concatenate(self.labels)
+
+
 class Trainer:
+    """policy: None for the gradient-threshold rule, or a ProposalNetwork whose logits choose the
+    densification at every density step. outcome_log: an OutcomeLog to fill with (features,
+    outcome label) samples; it labels each density step at the next one."""
+
     def __init__(self, context, params: np.ndarray, cameras: list, targets: list, scene_extent: float,
-                 background=(0.0, 0.0, 0.0), max_count: int | None = None, completed: int = 0, seed: int = 0):
+                 background=(0.0, 0.0, 0.0), max_count: int | None = None, completed: int = 0, seed: int = 0,
+                 policy=None, outcome_log: OutcomeLog | None = None, useful_opacity: float = 0.02):
         self.context = context
+        self.policy = policy
+        self.outcome_log = outcome_log
+        self.useful_opacity = useful_opacity
+        self.features_enabled = policy is not None or outcome_log is not None
+        self.pending = None  # the previous density step's decisions, kept for its outcome labels
         self.count = int(params.shape[1])
         self.params = DeviceBuffer.from_numpy(np.ascontiguousarray(params, np.float32))
         self.first = DeviceBuffer.zeros((58, self.count), np.float32)
@@ -59,6 +93,13 @@ class Trainer:
         self.density_counter = 0
         self.last_camera = -1
         self.history = []
+        self.feature_state = None
+        if self.features_enabled:
+            self._reset_feature_state()
+
+    def _reset_feature_state(self) -> None:
+        self.feature_state = DeviceBuffer((FEATURE_STATE_ROWS, max(self.count, 1)), np.float32)
+        call("gsn_density_features_clear", self.context.handle, self.feature_state.ptr, max(self.count, 1))
 
     # -- schedule ------------------------------------------------------------------------------
     def position_learning_rate(self, iteration: int) -> float:
@@ -95,6 +136,10 @@ class Trainer:
         if self.completed < 15000:
             call("gsn_density_accumulate", self.context.handle, self.statistics.ptr, self.absolute_statistics.ptr,
                  self.count, self.sums.ptr, self.absolute_sums.ptr, self.observations.ptr)
+            if self.features_enabled:
+                call("gsn_density_features_accumulate", self.context.handle, self.statistics.ptr,
+                     self.absolute_statistics.ptr, self.projection.depths.ptr, self.projection.conic_opacities.ptr,
+                     self.count, self.feature_state.ptr, max(self.count, 1))
             if self.completed > 500 and self.completed % 100 == 0:
                 self.control_density()
             if self.completed % 500 == 0:
@@ -103,6 +148,9 @@ class Trainer:
 
     # -- density -------------------------------------------------------------------------------
     def reset_opacity(self, cap: float) -> None:
+        if cap < 0.5:
+            # Opacities collapse: outcomes of the pending density step are not measurable.
+            self.pending = None
         logit = float(np.float32(math.log(np.float32(cap) / (np.float32(1.0) - np.float32(cap)))))
         call("gsn_density_reset_opacity", self.context.handle, self.params.ptr, self.count, self.count, logit)
         call("gsn_zero_rows", self.context.handle, self.first.ptr, self.second.ptr, self.count, 9, 1)
@@ -117,10 +165,28 @@ class Trainer:
         split_threshold = float(np.float32(percent_dense) * np.float32(self.scene_extent))
         world = float(np.float32(0.1) * np.float32(self.scene_extent))
 
+        features = None
+        if self.features_enabled:
+            features = DeviceBuffer((max(count, 1), FEATURES), np.float32)
+            call("gsn_density_features", self.context.handle, self.params.ptr, count, self.sums.ptr,
+                 self.absolute_sums.ptr, self.observations.ptr, self.feature_state.ptr, max(count, 1),
+                 gradient_threshold, split_threshold, self.scene_extent, features.ptr)
+        if self.outcome_log is not None and self.pending is not None:
+            self._log_outcomes(count)
+        logits = None
+        if self.policy is not None and count > 0:
+            logits = DeviceBuffer((count, 3), np.float32)
+            self.policy.forward(features, count, logits)
+
         def count_changes(threshold, absolute):
-            call("gsn_density_count", self.context.handle, self.params.ptr, count, self.sums.ptr,
-                 self.absolute_sums.ptr, self.observations.ptr, threshold, split_threshold, min_opacity, world,
-                 int(max_screen > 0), absolute, decisions.ptr, offsets.ptr, totals)
+            if logits is not None and np.isfinite(threshold):
+                call("gsn_density_count_proposed", self.context.handle, self.params.ptr, count, self.sums.ptr,
+                     self.absolute_sums.ptr, self.observations.ptr, logits.ptr, split_threshold, min_opacity, world,
+                     int(max_screen > 0), decisions.ptr, offsets.ptr, totals)
+            else:
+                call("gsn_density_count", self.context.handle, self.params.ptr, count, self.sums.ptr,
+                     self.absolute_sums.ptr, self.observations.ptr, threshold, split_threshold, min_opacity, world,
+                     int(max_screen > 0), absolute, decisions.ptr, offsets.ptr, totals)
             t = list(totals)
             return t, t[0] + t[1] + 2 * t[3]
 
@@ -145,9 +211,25 @@ class Trainer:
         self.absolute_sums = DeviceBuffer.zeros((max(out_count, 1),), np.float32)
         self.observations = DeviceBuffer.zeros((max(out_count, 1),), np.uint32)
         self.projection = None
+        if self.features_enabled:
+            self._reset_feature_state()
+        if self.outcome_log is not None:
+            # Rule labels of this step are judged at the next one from what its outputs became.
+            self.pending = (decisions, offsets, list(t), count, features.numpy()[:count] if count else None)
         changes = (t[1] + 2 * t[3], count - t[0])
         self.history.append((self.completed, count, out_count, changes))
         return changes
+
+    def _log_outcomes(self, count: int) -> None:
+        decisions, offsets, t, previous_count, previous_features = self.pending
+        self.pending = None
+        if previous_count == 0:
+            return
+        labels = DeviceBuffer((previous_count,), np.int32)
+        totals = (ctypes.c_int32 * 4)(*t)
+        call("gsn_density_outcomes", self.context.handle, decisions.ptr, offsets.ptr, totals, previous_count,
+             self.params.ptr, self.observations.ptr, count, self.useful_opacity, labels.ptr)
+        self.outcome_log.add(previous_features, labels.numpy())
 
     def params_host(self) -> np.ndarray:
         return self.params.numpy()[:, : self.count]

@@ -238,6 +238,48 @@ int gsn_density_reset_opacity(gsn_context* context, float* params, int stride, i
 int gsn_density_clear(gsn_context* context, float* gradient_sums,
     float* absolute_gradient_sums, uint32_t* observation_counts, int count);
 
+/* ---- learned densification inputs and policy (no counterpart in the reference) ----------- */
+
+/* gsn_density_count with the gradient-threshold selection replaced by a proposal: for each
+ * Gaussian the largest of logits[N][3] (keep, clone, split; ties and NaN favour keep, then clone)
+ * chooses the densification; pruning (opacity, oversize) stays the reference rule
+ * (density_controller.cu:95-115). Same outputs and synchronisation as gsn_density_count. */
+int gsn_density_count_proposed(gsn_context* context, const float* params, int count,
+    const float* gradient_sums, const float* absolute_gradient_sums,
+    const uint32_t* observation_counts, const float* proposal_logits, float split_threshold,
+    float min_opacity, float max_world_size, int prune_large, int32_t* decisions,
+    int32_t* offsets, int32_t* totals);
+
+/* Zeroes a [6][stride] feature state (max gradient, max absolute gradient, depth sum, depth min,
+ * depth max, max footprint radius). */
+int gsn_density_features_clear(gsn_context* context, float* state, int stride);
+
+/* Folds one view into the feature state of every observed Gaussian (statistics[1][id] != 0):
+ * statistics / absolute_statistics are the [2][N] outputs of the gradient pass, depths and
+ * conic_opacities the projection's. */
+int gsn_density_features_accumulate(gsn_context* context, const float* statistics,
+    const float* absolute_statistics, const float* depths, const float* conic_opacities, int count,
+    float* state, int stride);
+
+/* features[N][16] for the proposal network: gradient means / threshold (from the density
+ * accumulators, as the rule uses them), log1p(max gradients / threshold), log1p(observations),
+ * descending log-scales - log(split_threshold), opacity, log(depth mean/min/max / extent),
+ * SH DC, log1p(max footprint radius); observation-derived entries are 0 when unobserved. */
+int gsn_density_features(gsn_context* context, const float* params, int count,
+    const float* gradient_sums, const float* absolute_gradient_sums,
+    const uint32_t* observation_counts, const float* state, int stride,
+    float gradient_threshold, float split_threshold, float scene_extent, float* features);
+
+/* Training labels from the outcome of the previous density step. decisions/offsets/totals are
+ * that step's gsn_density_count outputs over previous_count Gaussians; params and
+ * observation_counts describe the count Gaussians it produced, after the interval that followed.
+ * labels[previous_count]: -1 pruned, 0 kept, 1 cloned with both copies useful, 2 split with both
+ * children useful (useful: observed in the interval and opacity >= min_opacity); a densified
+ * Gaussian whose outputs were not all useful is labelled 0. */
+int gsn_density_outcomes(gsn_context* context, const int32_t* decisions, const int32_t* offsets,
+    const int32_t* totals, int previous_count, const float* params,
+    const uint32_t* observation_counts, int count, float min_opacity, int32_t* labels);
+
 /* ---- initialisation ---------------------------------------------------------------------- */
 
 /* Three nearest neighbours of every point (ties broken by smaller index) through a linear BVH

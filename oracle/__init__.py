@@ -57,11 +57,12 @@ def _load() -> ctypes.CDLL:
         "oracle_render_f64": ([CAM, I, P, P, P, P, P, P, P, P, P], I),
         "oracle_loss_f64": ([P, P, I, I], ctypes.c_double),
         "oracle_image_gradients_f64": ([P, P, I, I, P], None),
-        "oracle_backward_f64": ([CAM, I, P, P, P, P, P, P, P, P, P, P, P], None),
+        "oracle_backward_f64": ([CAM, I, P, P, P, P, P, P, P, P, P, P, P, P], None),
         "oracle_map_f64": ([P, I, CAM, I, P, P, P, P, P], None),
         "oracle_adam_f32": ([I, P, P, P, P, F, F, F], None),
         "oracle_remap_f32": ([P, I, I, P, P, P, P], None),
         "oracle_density_count_f32": ([P, I, P, P, P, F, F, F, F, I, F, P, P], None),
+        "oracle_density_count_proposed_f32": ([P, I, P, F, F, F, I, P, P], None),
         "oracle_density_write_f32": ([P, I, P, P, ctypes.c_uint64, ctypes.c_uint64, P, I, P, P, P], None),
         "oracle_jitter_normal": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64], F),
         "oracle_philox4x32_10": ([P, ctypes.c_uint32, ctypes.c_uint32], None),
@@ -188,7 +189,8 @@ def image_gradients_f64(image: np.ndarray, target: np.ndarray) -> np.ndarray:
     return out
 
 
-def backward_f64(camera, projected: dict, image: np.ndarray, image_grad: np.ndarray) -> dict:
+def backward_f64(camera, projected: dict, image: np.ndarray, image_grad: np.ndarray,
+                 pixel_flags: np.ndarray | None = None) -> dict:
     cam = as_camera(camera)
     n = projected["depths"].shape[0]
     screen = np.zeros((11, n))
@@ -198,8 +200,9 @@ def backward_f64(camera, projected: dict, image: np.ndarray, image_grad: np.ndar
     image_grad = np.ascontiguousarray(image_grad, np.float64)
     lib.oracle_backward_f64(ctypes.byref(cam), n, _ptr(projected["means"]), _ptr(projected["conics"]),
                             _ptr(projected["colors"]), _ptr(projected["depths"]), _ptr(projected["tile_rects"]),
-                            _ptr(projected["visible"]), _ptr(image), _ptr(image_grad), _ptr(screen), _ptr(magnitude),
-                            _ptr(tainted))
+                            _ptr(projected["visible"]), _ptr(image), _ptr(image_grad),
+                            None if pixel_flags is None else _ptr(np.ascontiguousarray(pixel_flags, np.uint8)),
+                            _ptr(screen), _ptr(magnitude), _ptr(tainted))
     return {"screen": screen, "magnitude": magnitude, "tainted": tainted.astype(bool)}
 
 
@@ -228,7 +231,11 @@ def gradients_f64(params, camera, sh_degree, background, target, rects_from: dic
         projected["visible"] = (visible & (projected["visible"] > 0)).astype(np.uint8)
     image, ambiguous = render_f64(camera, projected, background)
     image_grad = image_gradients_f64(image, target)
-    back = backward_f64(camera, projected, image, image_grad)
+    # The L1 term's sign(image - target) is decided within FP32 rendering error (relative ~1e-5
+    # of the pixel value) at some pixels; an FP32 run may take the other sign there.
+    difference = np.abs(image - np.asarray(target, np.float64) / 255.0)
+    loss_ambiguous = np.any((difference <= 1e-5 * np.abs(image)) & (image != 0.0), axis=2)
+    back = backward_f64(camera, projected, image, image_grad, loss_ambiguous)
     observed = projected["visible"]
     mapped = map_f64(params, camera, sh_degree, back["screen"], observed)
     return {"projected": projected, "image": image, "ambiguous": ambiguous, "image_grad": image_grad, **back,
@@ -264,6 +271,16 @@ def density_count_f32(params, sums, abs_sums, counts, gradient_threshold, split_
                                  _ptr(np.ascontiguousarray(counts, np.uint32)), gradient_threshold, split_threshold,
                                  min_opacity, max_world_size, int(prune_large), absolute_gradient_threshold,
                                  _ptr(decisions), _ptr(offsets))
+    return decisions, offsets
+
+
+def density_count_proposed_f32(params, proposal, split_threshold, min_opacity, max_world_size, prune_large):
+    params = _f32(params)
+    n = params.shape[1]
+    decisions = np.zeros((n + 1, 4), np.int32)
+    offsets = np.zeros((n + 1, 4), np.int32)
+    lib.oracle_density_count_proposed_f32(_ptr(params), n, _ptr(_f32(proposal)), split_threshold, min_opacity,
+                                          max_world_size, int(prune_large), _ptr(decisions), _ptr(offsets))
     return decisions, offsets
 
 

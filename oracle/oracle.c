@@ -896,7 +896,8 @@ void oracle_image_gradients_f64(const double* image, const uint8_t* target, int 
 
 void oracle_backward_f64(const oracle_camera* cam, int count, const double* means, const double* conics,
     const double* colors, const double* depths, const int* rects, const uint8_t* visible, const double* image,
-    const double* image_gradients, double* screen, double* magnitude, uint8_t* tainted) {
+    const double* image_gradients, const uint8_t* pixel_flags, double* screen, double* magnitude,
+    uint8_t* tainted) {
     const int64_t n = count;
     memset(screen, 0, sizeof(double) * 11 * (size_t)count);
     memset(magnitude, 0, sizeof(double) * 11 * (size_t)count);
@@ -912,7 +913,12 @@ void oracle_backward_f64(const oracle_camera* cam, int count, const double* mean
             const double* final_color = image + ((int64_t)y * cam->width + x) * 3;
             const double* g = image_gradients + ((int64_t)y * cam->width + x) * 3;
             double T = 1.0, prefix[3] = {0.0, 0.0, 0.0};
-            int flag = 0, touched_count = 0;
+            /* Relative condition number of T = prod(1 - alpha_i) with respect to relative errors in
+             * the alphas: sum alpha_i / (1 - alpha_i). FP32 evaluations of T (and of everything
+             * built on it) carry errors proportional to it. */
+            double kappa = 0.0;
+            int flag = pixel_flags ? pixel_flags[(int64_t)y * cam->width + x] != 0 : 0;
+            int touched_count = 0;
             for (int64_t r = offsets[tile]; r < offsets[tile + 1]; ++r) {
                 int id = ids[r];
                 double dx = means[2 * id] - x, dy = means[2 * id + 1] - y;
@@ -957,11 +963,11 @@ void oracle_backward_f64(const oracle_camera* cam, int count, const double* mean
                         fabs(g[c]) * (fabs(prefix[c]) + fabs(T * col[c]) + fabs(final_color[c])) / (1.0 - alpha);
                     term = weight * g[c];
                     screen[(6 + c) * n + id] += term;
-                    magnitude[(6 + c) * n + id] += fabs(term);
+                    magnitude[(6 + c) * n + id] += fabs(term) * (1.0 + kappa);
                 }
                 if (raw_alpha < 0.99) {
                     double common = alpha_gradient * raw_alpha;
-                    double common_magnitude = alpha_magnitude * raw_alpha;
+                    double common_magnitude = alpha_magnitude * raw_alpha * (1.0 + kappa + alpha / (1.0 - alpha));
                     double qx = q[0] * dx + q[1] * dy;
                     double qy = q[1] * dx + q[2] * dy;
                     double terms[6] = {-common * qx, -common * qy, 0.5 * common * qx * qx, common * qx * qy,
@@ -981,6 +987,7 @@ void oracle_backward_f64(const oracle_camera* cam, int count, const double* mean
                     prefix[c] += col[c] * weight;
                 }
                 T = next;
+                kappa += alpha / (1.0 - alpha);
             }
             if (flag) {
                 for (int k = 0; k < touched_count; ++k) {
@@ -1200,24 +1207,35 @@ void oracle_remap_f32(const int* parent_ids, int previous_count, int count, cons
 
 /* ---- density control (density_controller.cu:60-278), FP32 mirror ------------------------- */
 
-void oracle_density_count_f32(const float* params, int count, const float* sums, const float* abs_sums,
+/* proposal == NULL: the reference rule. Otherwise logits[N][3] (keep, clone, split) choose the
+ * densification by the largest logit (ties and NaN favour keep, then clone); pruning is unchanged.
+ * The proposal form has no reference counterpart. */
+static void density_count(const float* params, int count, const float* sums, const float* abs_sums,
     const uint32_t* counts, float gradient_threshold, float split_threshold, float min_opacity, float max_world_size,
-    int prune_large, float absolute_gradient_threshold, int32_t* decisions, int32_t* offsets) {
+    int prune_large, float absolute_gradient_threshold, const float* proposal, int32_t* decisions, int32_t* offsets) {
     const int64_t n = count;
     for (int id = 0; id < count; ++id) {
         float opacity = 1.0f / (1.0f + oracle_expf(-params[OPACITY_INDEX * n + id]));
-        uint32_t observations = counts[id];
+        uint32_t observations = counts ? counts[id] : 0u;
         float gradient = observations ? sums[id] / (float)observations : 0.0f;
         if (gradient != gradient) gradient = 0.0f;
         float scale[3];
         for (int a = 0; a < 3; ++a) scale[a] = oracle_expf(params[(LOG_SCALE_INDEX + a) * n + id]);
         float max_scale = fmaxf(scale[0], fmaxf(scale[1], scale[2]));
         if (scale[0] != scale[0] || scale[1] != scale[1] || scale[2] != scale[2]) max_scale = NAN;
-        int selected = gradient >= gradient_threshold;
-        int clone = selected && max_scale <= split_threshold;
-        float absolute_gradient = observations ? abs_sums[id] / (float)observations : 0.0f;
-        int split_selected = absolute_gradient_threshold > 0.0f ? absolute_gradient >= absolute_gradient_threshold : selected;
-        int split = split_selected && max_scale > split_threshold;
+        int clone, split;
+        if (proposal) {
+            float k = proposal[3 * (int64_t)id], c = proposal[3 * (int64_t)id + 1], s = proposal[3 * (int64_t)id + 2];
+            clone = c > k && c >= s;
+            split = s > k && s > c;
+        } else {
+            int selected = gradient >= gradient_threshold;
+            clone = selected && max_scale <= split_threshold;
+            float absolute_gradient = observations ? abs_sums[id] / (float)observations : 0.0f;
+            int split_selected =
+                absolute_gradient_threshold > 0.0f ? absolute_gradient >= absolute_gradient_threshold : selected;
+            split = split_selected && max_scale > split_threshold;
+        }
         float remaining = max_scale;
         if (split && prune_large) {
             float x = oracle_expf(oracle_logf(scale[0] / 1.6f));
@@ -1240,6 +1258,19 @@ void oracle_density_count_f32(const float* params, int count, const float* sums,
             running[k] += decisions[4 * id + k];
         }
     }
+}
+
+void oracle_density_count_f32(const float* params, int count, const float* sums, const float* abs_sums,
+    const uint32_t* counts, float gradient_threshold, float split_threshold, float min_opacity, float max_world_size,
+    int prune_large, float absolute_gradient_threshold, int32_t* decisions, int32_t* offsets) {
+    density_count(params, count, sums, abs_sums, counts, gradient_threshold, split_threshold, min_opacity,
+        max_world_size, prune_large, absolute_gradient_threshold, NULL, decisions, offsets);
+}
+
+void oracle_density_count_proposed_f32(const float* params, int count, const float* proposal, float split_threshold,
+    float min_opacity, float max_world_size, int prune_large, int32_t* decisions, int32_t* offsets) {
+    density_count(params, count, NULL, NULL, NULL, 0.0f, split_threshold, min_opacity, max_world_size, prune_large,
+        0.0f, proposal, decisions, offsets);
 }
 
 void oracle_philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {

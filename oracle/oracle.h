@@ -77,12 +77,16 @@ void oracle_image_gradients_f64(const double* image, const uint8_t* target, int 
 
 /* solver.cu:325-466 in FP64: per-pixel front-to-back replay from the final colour. screen[11][N]
  * receives dmean2D(2), dSigma(3), dlogit, dcolor(3), |dmean2D|(2); magnitude[11][N] the sums of
- * the per-pixel terms' error-bound magnitudes (for the geometry fields, dL/dalpha's summands
- * |g| (|prefix| + |T c| + |C|) / (1 - alpha) in place of the cancelled |dL/dalpha|), the scale
- * for tolerances; tainted[N] marks Gaussians that touch an ambiguous pixel. */
+ * the per-pixel terms' error-bound magnitudes, the scale for tolerances: |term| (1 + kappa) with
+ * kappa = sum over the splats in front of alpha / (1 - alpha), the relative condition number of
+ * T; for the geometry fields dL/dalpha's summands |g| (|prefix| + |T c| + |C|) / (1 - alpha) stand
+ * in for the cancelled |dL/dalpha| and the splat's own alpha / (1 - alpha) joins kappa.
+ * tainted[N] marks Gaussians that touch an ambiguous pixel; pixel_flags[H][W] (optional) adds
+ * pixels the caller knows to be ambiguous (e.g. an L1 sign decided within FP32 error). */
 void oracle_backward_f64(const oracle_camera* camera, int count, const double* means, const double* conics,
     const double* colors, const double* depths, const int* tile_rects, const uint8_t* visible, const double* image,
-    const double* image_gradients, double* screen, double* magnitude, uint8_t* tainted);
+    const double* image_gradients, const uint8_t* pixel_flags, double* screen, double* magnitude,
+    uint8_t* tainted);
 
 /* solver.cu:468-490 and 492-623 in FP64: statistics[2][N], absolute_statistics[2][N] and the
  * parameter gradients[58][N] from screen[11][N] and record counts (observed = count > 0). */
@@ -102,6 +106,13 @@ void oracle_density_count_f32(const float* params, int count, const float* gradi
     const float* absolute_gradient_sums, const uint32_t* observation_counts, float gradient_threshold,
     float split_threshold, float min_opacity, float max_world_size, int prune_large,
     float absolute_gradient_threshold, int32_t* decisions, int32_t* offsets);
+
+/* The same with a learned proposal: logits[N][3] (keep, clone, split) replace the gradient rule's
+ * selection by their largest entry (ties and NaN favour keep, then clone); pruning unchanged.
+ * No reference counterpart. */
+void oracle_density_count_proposed_f32(const float* params, int count, const float* proposal,
+    float split_threshold, float min_opacity, float max_world_size, int prune_large, int32_t* decisions,
+    int32_t* offsets);
 
 /* density_controller.cu:168-278 with the build's Philox4x32-10 + Box-Muller jitter. */
 void oracle_density_write_f32(const float* params, int count, const int32_t* decisions, const int32_t* offsets,

@@ -3,10 +3,14 @@ history remap against the oracle, plus the reference's own solver cases (test/so
 
 Tolerance rule for gradients (sums over many pixels cancel): |gpu - oracle| <= 1e-5 * sum|terms|
 + 1e-9 * max|row|, where sum|terms| is the oracle's sum of the per-pixel terms' error-bound
-magnitudes. For the geometry fields that magnitude is built from |g| (|prefix| + |T c| + |C|) /
-(1 - alpha), the summands of dL/dalpha before they cancel: the replay from the final colour
-(solver.cu:389-396) subtracts nearly equal FP32 colours behind the last splats of a pixel, so its
-error is relative to those summands, as it is in the reference's own FP32 kernel.
+magnitudes, i.e. 1e-5 relative to what an FP32 evaluation of the same algorithm can resolve:
+  - every term is weighted by 1 + kappa, kappa = sum alpha_i / (1 - alpha_i) over the splats in
+    front: the relative condition number of T = prod(1 - alpha_i) (1 - alpha loses bits as alpha
+    nears the 0.99 cap);
+  - for the geometry fields, |g| (|prefix| + |T c| + |C|) / (1 - alpha), the summands of dL/dalpha
+    before they cancel, stands in for |dL/dalpha|: the replay from the final colour
+    (solver.cu:389-396) subtracts nearly equal FP32 colours behind the last splats of a pixel.
+The reference's own FP32 kernel has the same error behaviour.
 """
 import json
 import pathlib
