@@ -151,7 +151,7 @@ def main() -> int:
     parser.add_argument("--gaussians", type=int, default=3_000_000)
     parser.add_argument("--width", type=int, default=1920)
     parser.add_argument("--height", type=int, default=1080)
-    parser.add_argument("--cameras", type=int, default=32)
+    parser.add_argument("--cameras", type=int, default=300)
     parser.add_argument("--start-iteration", type=int, default=3000)
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
     parser.add_argument("--no-cpu-baseline", action="store_true")

@@ -124,6 +124,11 @@ _PROTOTYPES = {
     "gsn_density_features_accumulate": [_P, _P, _P, _P, _P, _I, _P, _I],
     "gsn_density_features": [_P, _P, _I, _P, _P, _P, _P, _I, _F, _F, _F, _P],
     "gsn_density_outcomes": [_P, _P, _P, ctypes.POINTER(ctypes.c_int32), _I, _P, _P, _I, _F, _P],
+    "gsn_render_tiles_stored": [_P, _CAM, _P, _P, _P, _P, _P, ctypes.POINTER(_F), _P, _P, _P],
+    "gsn_gaussian_gradients_stored": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
+                                      ctypes.POINTER(_F), _P, _P, _P, _P, _P],
+    "gsn_gaussian_gradients_adam_stored": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
+                                           ctypes.POINTER(_F), _P, _P, _P, _F, _F, _F, _P, _P],
     "gsn_mlp_sizes": [ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "gsn_mlp_sync_weights": [_P, _P, _P],
     "gsn_mlp_forward": [_P, _P, _I, _P, _P, _P],
@@ -182,6 +187,13 @@ def device_name() -> str:
     buffer = ctypes.create_string_buffer(256)
     call("gsn_device_name", buffer, 256)
     return buffer.value.decode()
+
+
+def device_memory_info() -> tuple[int, int]:
+    """(free, total) bytes of the current device."""
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    call("gsn_device_memory_info", ctypes.byref(free), ctypes.byref(total))
+    return free.value, total.value
 
 
 class DeviceBuffer:
@@ -448,3 +460,26 @@ class ProposalNetwork:
              _p(self.params), _p(self.first), _p(self.second), _p(self.weights), learning_rate, self.step,
              _p(gradients), ctypes.byref(loss) if want_loss else None)
         return loss.value if want_loss else None
+
+
+def render_stored(context: Context, projection: Projection, image: DeviceBuffer, background,
+                  final_transmittance: DeviceBuffer, last_contributor: DeviceBuffer) -> None:
+    """Re-runs the raster forward of an existing projection with the per-pixel state the
+    stored-state backward needs (final transmittance, last contributor)."""
+    p = projection
+    bg = (ctypes.c_float * 3)(*background)
+    call("gsn_render_tiles_stored", context.handle, ctypes.byref(p.camera), _p(p.tile_gaussian_ids),
+         _p(p.tile_offsets), _p(p.means), _p(p.conic_opacities), _p(p.colors), bg, _p(image),
+         _p(final_transmittance), _p(last_contributor))
+
+
+def gaussian_gradients_stored(context: Context, projection: Projection, params: DeviceBuffer, sh_degree: int,
+                              final_transmittance: DeviceBuffer, last_contributor: DeviceBuffer, background,
+                              image_grad: DeviceBuffer, gradients: DeviceBuffer | None, statistics: DeviceBuffer,
+                              absolute_statistics: DeviceBuffer, screen: DeviceBuffer | None = None) -> None:
+    p = projection
+    bg = (ctypes.c_float * 3)(*background)
+    call("gsn_gaussian_gradients_stored", context.handle, ctypes.byref(p.camera), p.count, sh_degree, _p(params),
+         _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets), _p(p.tile_rects),
+         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(final_transmittance), _p(last_contributor), bg,
+         _p(image_grad), _p(gradients), _p(screen), _p(statistics), _p(absolute_statistics))

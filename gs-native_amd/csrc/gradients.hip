@@ -247,15 +247,6 @@ __device__ __forceinline__ void map_gaussian(const float* __restrict__ params, c
     out.grad[kOpacityIndex] = in[5];
 }
 
-struct StoreSink {
-    float* gradients;
-    int64_t n;
-    int id;
-    __device__ __forceinline__ void operator()(int row, float g) const {
-        gradients[row * n + id] = g;
-    }
-};
-
 // Dense Adam step of one entry (reference solver.cu:666-688); no FP contraction so that the
 // result equals the host restatement bit for bit for equal inputs.
 __device__ __forceinline__ float learning_rate_for_row(int row, float position_learning_rate) {
@@ -274,28 +265,26 @@ __device__ __forceinline__ float learning_rate_for_row(int row, float position_l
     return (row - kShIndex) % 16 == 0 ? 0.0025f : 0.0025f / 20.0f;
 }
 
-__device__ __forceinline__ void adam_entry(float* params, float* first, float* second, int64_t index, float gradient,
+__device__ __forceinline__ void adam_values(float& param, float& first, float& second, float gradient,
     float learning_rate, float first_correction, float second_correction) {
 #pragma clang fp contract(off)
-    float m = 0.9f * first[index] + 0.1f * gradient;
-    float v = 0.999f * second[index] + 0.001f * gradient * gradient;
-    first[index] = m;
-    second[index] = v;
-    params[index] -= learning_rate * (m * first_correction) / (sqrtf(v * second_correction) + 1e-15f);
+    float m = 0.9f * first + 0.1f * gradient;
+    float v = 0.999f * second + 0.001f * gradient * gradient;
+    first = m;
+    second = v;
+    param -= learning_rate * (m * first_correction) / (sqrtf(v * second_correction) + 1e-15f);
 }
 
-struct AdamSink {
-    float* params;
-    float* first;
-    float* second;
-    int64_t n;
-    int id;
-    float position_learning_rate, first_correction, second_correction;
-    __device__ __forceinline__ void operator()(int row, float g) const {
-        adam_entry(params, first, second, row * n + id, g, learning_rate_for_row(row, position_learning_rate),
-            first_correction, second_correction);
-    }
-};
+__device__ __forceinline__ void adam_entry(float* params, float* first, float* second, int64_t index, float gradient,
+    float learning_rate, float first_correction, float second_correction) {
+    float p = params[index], m = first[index], v = second[index];
+    adam_values(p, m, v, gradient, learning_rate, first_correction, second_correction);
+    first[index] = m;
+    second[index] = v;
+    params[index] = p;
+}
+
+constexpr int kAdamGroup = 8;
 
 constexpr int kGradientBlock = 256;
 constexpr int kGradientChunk = 256;  // partial records staged per LDS round (12 KiB)
@@ -377,30 +366,57 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
         map_gaussian<Degree>(params, cam, id, n, in, mapped);
     }
     constexpr int coefficients = (Degree + 1) * (Degree + 1);
-    auto emit = [&](auto& sink) {
-#pragma unroll
-        for (int row = 0; row < kParamCount; ++row) {
-            float g = 0.0f;
-            if (active) {
-                if (row < kShIndex) {
-                    g = mapped.grad[row];
-                }
-                else {
-                    int channel = (row - kShIndex) / 16;
-                    int k = (row - kShIndex) % 16;
-                    g = (k < coefficients && mapped.color_ok[channel]) ? in[6 + channel] * mapped.basis[k] : 0.0f;
-                }
+    auto row_gradient = [&](int row) {
+        float g = 0.0f;
+        if (active) {
+            if (row < kShIndex) {
+                g = mapped.grad[row];
             }
-            sink(row, g);
+            else {
+                int channel = (row - kShIndex) / 16;
+                int k = (row - kShIndex) % 16;
+                g = (k < coefficients && mapped.color_ok[channel]) ? in[6 + channel] * mapped.basis[k] : 0.0f;
+            }
         }
+        return g;
     };
     if (Fused) {
-        AdamSink sink{params, first, second, n, id, position_learning_rate, first_correction, second_correction};
-        emit(sink);
+        // Adam over the 58 rows in groups: all loads of a group are issued before its stores, so
+        // each wave keeps 3 * kAdamGroup loads in flight instead of one row's.
+        float* __restrict__ p = params;
+        float* __restrict__ m = first;
+        float* __restrict__ v = second;
+#pragma unroll
+        for (int base = 0; base < kParamCount; base += kAdamGroup) {
+            float pv[kAdamGroup], mv[kAdamGroup], vv[kAdamGroup];
+#pragma unroll
+            for (int k = 0; k < kAdamGroup; ++k) {
+                if (base + k < kParamCount) {
+                    int64_t index = (base + k) * n + id;
+                    pv[k] = p[index];
+                    mv[k] = m[index];
+                    vv[k] = v[index];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kAdamGroup; ++k) {
+                if (base + k < kParamCount) {
+                    int row = base + k;
+                    int64_t index = row * n + id;
+                    adam_values(pv[k], mv[k], vv[k], row_gradient(row),
+                        learning_rate_for_row(row, position_learning_rate), first_correction, second_correction);
+                    p[index] = pv[k];
+                    m[index] = mv[k];
+                    v[index] = vv[k];
+                }
+            }
+        }
     }
     else if (gradients) {
-        StoreSink sink{gradients, n, id};
-        emit(sink);
+#pragma unroll
+        for (int row = 0; row < kParamCount; ++row) {
+            gradients[row * n + id] = row_gradient(row);
+        }
     }
 }
 
@@ -515,8 +531,8 @@ extern "C" int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* ca
         return GSN_ERROR_OUT_OF_MEMORY;
     }
     if (record_count > 0) {
-        GSN_TRY(launch_splat_backward(context, cam, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors,
-            tile_rects, record_offsets, image, image_gradients, partials, record_count));
+        GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
+            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count));
     }
     return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets, const_cast<float*>(params),
         gradients, screen_gradients, statistics, absolute_statistics, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
@@ -546,12 +562,80 @@ extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camer
         return GSN_ERROR_OUT_OF_MEMORY;
     }
     if (record_count > 0) {
-        GSN_TRY(launch_splat_backward(context, cam, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors,
-            tile_rects, record_offsets, image, image_gradients, partials, record_count));
+        GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
+            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count));
     }
     return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, params, nullptr, nullptr,
         statistics, absolute_statistics, first_moments, second_moments, position_learning_rate, first_correction,
         second_correction);
+}
+
+extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_camera* camera, int count,
+    int sh_degree, const float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
+    const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
+    const float* colors, const float* final_transmittance, const int32_t* last_contributor, const float* background,
+    const float* image_gradients, float* gradients, float* screen_gradients, float* statistics,
+    float* absolute_statistics) {
+    using namespace gsn;
+    // The image itself is not read by this form; validation only needs a non-null stand-in.
+    GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
+        record_offsets, tile_rects, means, conic_opacities, colors, image_gradients, image_gradients, statistics,
+        absolute_statistics));
+    if (count == 0) {
+        return GSN_OK;
+    }
+    if (!final_transmittance || !last_contributor || !background) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_gaussian_gradients_stored: null forward state");
+    }
+    Camera cam;
+    memcpy(&cam, camera, sizeof(cam));
+    float* partials = static_cast<float*>(
+        scratch(context, kSlotPartials, static_cast<size_t>(record_count > 0 ? record_count : 1) * kPartialFloats * 4));
+    if (!partials) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    if (record_count > 0) {
+        StoredState stored{final_transmittance, last_contributor, {background[0], background[1], background[2]}};
+        GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
+            colors, tile_rects, record_offsets, nullptr, image_gradients, partials, record_count, &stored));
+    }
+    return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets,
+        const_cast<float*>(params), gradients, screen_gradients, statistics, absolute_statistics, nullptr, nullptr,
+        0.0f, 0.0f, 0.0f);
+}
+
+extern "C" int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gsn_camera* camera, int count,
+    int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
+    const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
+    const float* colors, const float* final_transmittance, const int32_t* last_contributor, const float* background,
+    const float* image_gradients, float* first_moments, float* second_moments, float position_learning_rate,
+    float first_correction, float second_correction, float* statistics, float* absolute_statistics) {
+    using namespace gsn;
+    GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
+        record_offsets, tile_rects, means, conic_opacities, colors, image_gradients, image_gradients, statistics,
+        absolute_statistics));
+    if (count == 0) {
+        return GSN_OK;
+    }
+    if (!final_transmittance || !last_contributor || !background || !first_moments || !second_moments ||
+        !(position_learning_rate >= 0.0f) || !std::isfinite(position_learning_rate)) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_gaussian_gradients_adam_stored: invalid arguments");
+    }
+    Camera cam;
+    memcpy(&cam, camera, sizeof(cam));
+    float* partials = static_cast<float*>(
+        scratch(context, kSlotPartials, static_cast<size_t>(record_count > 0 ? record_count : 1) * kPartialFloats * 4));
+    if (!partials) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    if (record_count > 0) {
+        StoredState stored{final_transmittance, last_contributor, {background[0], background[1], background[2]}};
+        GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
+            colors, tile_rects, record_offsets, nullptr, image_gradients, partials, record_count, &stored));
+    }
+    return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, params,
+        nullptr, nullptr, statistics, absolute_statistics, first_moments, second_moments, position_learning_rate,
+        first_correction, second_correction);
 }
 
 extern "C" int gsn_adam_update(gsn_context* context, int count, const float* gradients, float* params,

@@ -32,6 +32,7 @@ enum ScratchSlot {
     kSlotKnnNodes,
     kSlotKnnPoints,
     kSlotMisc,
+    kSlotSplats,
     kSlotCount
 };
 
@@ -110,10 +111,19 @@ int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int
 // |dmean2D|(2), pad.
 constexpr int kPartialFloats = 12;
 
-// Tile rasteriser backward writing per-record partials (raster_backward.hip).
-int launch_splat_backward(gsn_context* context, const Camera& cam, const int* ids, const int64_t* tile_offsets,
-    const float* means, const float* conics, const float* colors, const int* rects, const int* record_offsets,
-    const float* image, const float* image_gradients, float* partials, int record_count);
+// Per-pixel forward state consumed by the stored-state backward.
+struct StoredState {
+    const float* final_transmittance;
+    const int* last_contributor;
+    float background[3];
+};
+
+// Tile rasteriser backward writing per-record partials (raster_backward.hip): the recompute form
+// when stored is null, else the back-to-front stored-state form.
+int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
+    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
+    const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
+    const StoredState* stored = nullptr);
 
 inline unsigned grid_for(int64_t n, int block) {
     return static_cast<unsigned>((n + block - 1) / block);

@@ -18,17 +18,51 @@ namespace gsn {
 
 namespace {
 
-__device__ __forceinline__ int64_t partial_slot(int id, int tile_x, int tile_y, const int4* rects,
-    const int* record_offsets) {
+typedef float float4_vector __attribute__((ext_vector_type(4)));
+
+// Everything the backward needs of one Gaussian in 48 bytes, so that each record costs one
+// gather instead of five (means, conics, colours, rectangles, record offsets):
+//   a = (mean x, mean y, conic A, conic B), b = (conic C, opacity, red, green),
+//   c = (blue, record offset, rectangle x0 | y0 << 16, rectangle width) (last three as int bits).
+struct alignas(16) PackedSplat {
+    float4 a, b, c;
+};
+
+__global__ __launch_bounds__(256) void pack_splats_kernel(const float2* __restrict__ means,
+    const float4* __restrict__ conics, const float* __restrict__ colors, const int4* __restrict__ rects,
+    const int* __restrict__ record_offsets, int count, PackedSplat* __restrict__ packed) {
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count || record_offsets[id + 1] == record_offsets[id]) {
+        return;  // culled: never referenced by a record
+    }
+    float2 mean = means[id];
+    float4 conic = conics[id];
     int4 rect = rects[id];
-    int width = rect.z - rect.x;
-    return static_cast<int64_t>(record_offsets[id]) + (tile_y - rect.y) * width + (tile_x - rect.x);
+    const float* color = colors + static_cast<int64_t>(id) * 3;
+    PackedSplat out;
+    out.a = make_float4(mean.x, mean.y, conic.x, conic.y);
+    out.b = make_float4(conic.z, conic.w, color[0], color[1]);
+    out.c = make_float4(color[2], __int_as_float(record_offsets[id]), __int_as_float(rect.x | (rect.y << 16)),
+        __int_as_float(rect.z - rect.x));
+    packed[id] = out;
+}
+
+__device__ __forceinline__ int64_t partial_slot(float4 c, int tile_x, int tile_y) {
+    int origin = __float_as_int(c.z);
+    int width = __float_as_int(c.w);
+    return static_cast<int64_t>(__float_as_int(c.y)) + (tile_y - (origin >> 16)) * width + (tile_x - (origin & 0xffff));
+}
+
+// Partials are written once and read once by the per-Gaussian pass: non-temporal stores keep
+// them from displacing the Gaussian records that later tiles gather.
+__device__ __forceinline__ void store_partial(float* destination, const float4& value) {
+    __builtin_nontemporal_store(float4_vector{value.x, value.y, value.z, value.w},
+        reinterpret_cast<float4_vector*>(destination));
 }
 
 __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restrict__ ids,
-    const int64_t* __restrict__ tile_offsets, const float2* __restrict__ means, const float4* __restrict__ conics,
-    const float* __restrict__ colors, const int4* __restrict__ rects, const int* __restrict__ record_offsets,
-    Camera cam, const float* __restrict__ image, const float* __restrict__ image_gradients, float* __restrict__ partials,
+    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats, Camera cam,
+    const float* __restrict__ image, const float* __restrict__ image_gradients, float* __restrict__ partials,
     int tile_count, int record_count) {
     __shared__ float s_partial[64 * kPartialFloats];
     const int tile = xcd_remap(blockIdx.x, tile_count);
@@ -78,17 +112,15 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
         int mask = 0;
         int64_t slot = 0;
         if (valid) {
-            int id = ids[record];
-            float2 mean = means[id];
-            conic = conics[id];
-            const float* color = colors + static_cast<int64_t>(id) * 3;
-            sr = color[0];
-            sg = color[1];
-            sb = color[2];
-            mx = mean.x;
-            my = mean.y;
+            const PackedSplat splat = splats[ids[record]];
+            mx = splat.a.x;
+            my = splat.a.y;
+            conic = make_float4(splat.a.z, splat.a.w, splat.b.x, splat.b.y);
+            sr = splat.b.z;
+            sg = splat.b.w;
+            sb = splat.c.x;
             mask = band_mask(mx, my, conic, x0, y0);
-            slot = partial_slot(id, tile_x, tile_y, rects, record_offsets);
+            slot = partial_slot(splat.c, tile_x, tile_y);
         }
 #pragma unroll
         for (int k = 0; k < kPartialFloats; ++k) {
@@ -174,10 +206,10 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
         __syncthreads();
         if (valid && slot >= 0 && slot < record_count) {
             const float4* source = reinterpret_cast<const float4*>(s_partial + lane * kPartialFloats);
-            float4* destination = reinterpret_cast<float4*>(partials + slot * kPartialFloats);
-            destination[0] = source[0];
-            destination[1] = source[1];
-            destination[2] = source[2];
+            float* destination = partials + slot * kPartialFloats;
+            store_partial(destination, source[0]);
+            store_partial(destination + 4, source[1]);
+            store_partial(destination + 8, source[2]);
         }
         __syncthreads();
     }
@@ -185,16 +217,190 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
     for (; base < end; base += 64) {
         const int64_t record = base + lane;
         if (record < end) {
-            int id = ids[record];
-            int64_t slot = partial_slot(id, tile_x, tile_y, rects, record_offsets);
+            int64_t slot = partial_slot(splats[ids[record]].c, tile_x, tile_y);
             if (slot < 0 || slot >= record_count) {
                 continue;
             }
-            float4* destination = reinterpret_cast<float4*>(partials + slot * kPartialFloats);
+            float* destination = partials + slot * kPartialFloats;
             float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            destination[0] = zero;
-            destination[1] = zero;
-            destination[2] = zero;
+            store_partial(destination, zero);
+            store_partial(destination + 4, zero);
+            store_partial(destination + 8, zero);
+        }
+    }
+}
+
+// Stored-state variant (no counterpart in the reference; kept for the memory comparison with the
+// recompute form above). The forward stored, per pixel, the final transmittance T_N and the
+// tile-relative index + 1 of the last blended splat. The pass walks each tile's records back to
+// front from the last contributor of any of its pixels, recovering T_k = T_{k+1} / (1 - alpha_k)
+// and the colour behind splat k, B_k = T_N * background + sum_{j > k} c_j alpha_j T_j, so that
+//   dL/dalpha_k = sum_c g_c (T_k c_c - B_c / (1 - alpha_k)),
+// the same quantity as the recompute form's sum_c g_c (prefix_c + T_k c_c - C_c) / (1 - alpha_k).
+// Partials go to the same per-record slots.
+__global__ __launch_bounds__(64) void raster_backward_stored_kernel(const int* __restrict__ ids,
+    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats, Camera cam,
+    const float* __restrict__ final_transmittance, const int* __restrict__ last_contributor,
+    const float* __restrict__ image_gradients, float bg_r, float bg_g, float bg_b, float* __restrict__ partials,
+    int tile_count, int record_count) {
+    __shared__ float s_partial[64 * kPartialFloats];
+    const int tile = xcd_remap(blockIdx.x, tile_count);
+    const int lane = threadIdx.x;
+    const int tile_x = tile % cam.tile_count_x;
+    const int tile_y = tile / cam.tile_count_x;
+    const int x0 = tile_x * kTileSize;
+    const int y0 = tile_y * kTileSize;
+    const int px = x0 + (lane & 15);
+    const int py = y0 + (lane >> 4);
+    const float fx = static_cast<float>(px);
+    const float background[3] = {bg_r, bg_g, bg_b};
+    float fy[4], T[4], behind[4][3], grad[4][3];
+    int last[4];
+    int tile_last = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        int y = py + 4 * b;
+        bool inside = px < cam.width && y < cam.height;
+        fy[b] = static_cast<float>(y);
+        int64_t pixel = static_cast<int64_t>(y) * cam.width + px;
+        T[b] = inside ? final_transmittance[pixel] : 1.0f;
+        last[b] = inside ? last_contributor[pixel] : 0;
+        tile_last = max(tile_last, last[b]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            grad[b][c] = inside ? image_gradients[pixel * 3 + c] : 0.0f;
+            behind[b][c] = T[b] * background[c];
+        }
+    }
+    // Largest last contributor over the tile (wave maximum through LDS-free lane reads).
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        tile_last = max(tile_last, __shfl_xor(tile_last, offset));
+    }
+    const int64_t begin = tile_offsets[tile];
+    const int64_t end = tile_offsets[tile + 1];
+    const int64_t stop = min(end, begin + static_cast<int64_t>(tile_last));
+    for (int64_t top = stop; top > begin; top -= 64) {
+        const int64_t base = max(begin, top - 64);
+        const int64_t record = base + lane;
+        const bool valid = record < top;
+        float mx = 0.0f, my = 0.0f, sr = 0.0f, sg = 0.0f, sb = 0.0f;
+        float4 conic = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        int mask = 0;
+        int64_t slot = 0;
+        if (valid) {
+            const PackedSplat splat = splats[ids[record]];
+            mx = splat.a.x;
+            my = splat.a.y;
+            conic = make_float4(splat.a.z, splat.a.w, splat.b.x, splat.b.y);
+            sr = splat.b.z;
+            sg = splat.b.w;
+            sb = splat.c.x;
+            mask = band_mask(mx, my, conic, x0, y0);
+            slot = partial_slot(splat.c, tile_x, tile_y);
+        }
+#pragma unroll
+        for (int k = 0; k < kPartialFloats; ++k) {
+            s_partial[lane * kPartialFloats + k] = 0.0f;
+        }
+        __syncthreads();
+        uint64_t pending = __ballot(mask != 0);
+        while (pending) {
+            const int j = 63 - __builtin_clzll(pending);
+            pending &= ~(1ull << j);
+            const int position = static_cast<int>(base - begin) + j;  // tile-relative record index
+            const int smask = readlane_i(mask, j);
+            const float sx = readlane_f(mx, j);
+            const float sy = readlane_f(my, j);
+            const float ca = readlane_f(conic.x, j);
+            const float cbv = readlane_f(conic.y, j);
+            const float cc = readlane_f(conic.z, j);
+            const float op = readlane_f(conic.w, j);
+            const float col[3] = {readlane_f(sr, j), readlane_f(sg, j), readlane_f(sb, j)};
+            const float dx = sx - fx;
+            float acc[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                acc[k] = 0.0f;
+            }
+            bool contributed = false;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if (smask & (1 << b)) {
+                    float dy = sy - fy[b];
+                    float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbv * dx * dy;
+                    float raw_alpha = op * __expf(power);
+                    float alpha = fminf(kAlphaMax, raw_alpha);
+                    bool blend = position < last[b] && !(power > 0.0f) && alpha >= kAlphaMin;
+                    if (blend) {
+                        contributed = true;
+                        float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
+                        float before = T[b] * inv;  // T_k, transmittance in front of this splat
+                        float weight = alpha * before;
+                        float alpha_gradient = 0.0f;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            alpha_gradient += grad[b][c] * (before * col[c] - behind[b][c] * inv);
+                            acc[6 + c] += weight * grad[b][c];
+                        }
+                        if (raw_alpha < kAlphaMax) {
+                            float common = alpha_gradient * raw_alpha;
+                            float qx = ca * dx + cbv * dy;
+                            float qy = cbv * dx + cc * dy;
+                            float gx = -common * qx;
+                            float gy = -common * qy;
+                            acc[0] += gx;
+                            acc[1] += gy;
+                            acc[2] += 0.5f * common * qx * qx;
+                            acc[3] += common * qx * qy;
+                            acc[4] += 0.5f * common * qy * qy;
+                            acc[5] += common * (1.0f - op);
+                            acc[9] += fabsf(gx);
+                            acc[10] += fabsf(gy);
+                        }
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            behind[b][c] += col[c] * weight;
+                        }
+                        T[b] = before;
+                    }
+                }
+            }
+            if (__ballot(contributed)) {
+                float sums[3];
+                wave_sum12(acc, sums);
+                if ((lane & 15) == 0) {
+                    int row = lane >> 4;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        s_partial[j * kPartialFloats + 3 * row + k] = sums[k];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (valid && slot >= 0 && slot < record_count) {
+            const float4* source = reinterpret_cast<const float4*>(s_partial + lane * kPartialFloats);
+            float* destination = partials + slot * kPartialFloats;
+            store_partial(destination, source[0]);
+            store_partial(destination + 4, source[1]);
+            store_partial(destination + 8, source[2]);
+        }
+        __syncthreads();
+    }
+    // Records behind every pixel's last contributor contribute nothing.
+    for (int64_t base = stop; base < end; base += 64) {
+        const int64_t record = base + lane;
+        if (record < end) {
+            int64_t slot = partial_slot(splats[ids[record]].c, tile_x, tile_y);
+            if (slot < 0 || slot >= record_count) {
+                continue;
+            }
+            float* destination = partials + slot * kPartialFloats;
+            float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            store_partial(destination, zero);
+            store_partial(destination + 4, zero);
+            store_partial(destination + 8, zero);
         }
     }
 }
@@ -216,15 +422,33 @@ __global__ __launch_bounds__(64) void wave_sum12_selftest_kernel(const float* in
 
 } // namespace
 
-int launch_splat_backward(gsn_context* context, const Camera& cam, const int* ids, const int64_t* tile_offsets,
-    const float* means, const float* conics, const float* colors, const int* rects, const int* record_offsets,
-    const float* image, const float* image_gradients, float* partials, int record_count) {
+int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
+    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
+    const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
+    const StoredState* stored) {
+    if (cam.tile_count_x > 0xffff || cam.tile_count_y > 0x7fff) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "raster backward: tile grid exceeds 65535 x 32767");
+    }
     int64_t tiles = static_cast<int64_t>(cam.tile_count_x) * cam.tile_count_y;
+    PackedSplat* splats = static_cast<PackedSplat*>(
+        scratch(context, kSlotSplats, static_cast<size_t>(count > 0 ? count : 1) * sizeof(PackedSplat)));
+    if (!splats) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
     StageScope scope(context, kStageRasterBackward);
-    raster_backward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets,
-        reinterpret_cast<const float2*>(means), reinterpret_cast<const float4*>(conics), colors,
-        reinterpret_cast<const int4*>(rects), record_offsets, cam, image, image_gradients, partials,
-        static_cast<int>(tiles), record_count);
+    pack_splats_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(reinterpret_cast<const float2*>(means),
+        reinterpret_cast<const float4*>(conics), colors, reinterpret_cast<const int4*>(rects), record_offsets, count,
+        splats);
+    GSN_TRY(check_launch("pack_splats_kernel"));
+    if (stored) {
+        raster_backward_stored_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets,
+            splats, cam, stored->final_transmittance, stored->last_contributor, image_gradients,
+            stored->background[0], stored->background[1], stored->background[2], partials, static_cast<int>(tiles),
+            record_count);
+        return check_launch("raster_backward_stored_kernel");
+    }
+    raster_backward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets, splats, cam,
+        image, image_gradients, partials, static_cast<int>(tiles), record_count);
     return check_launch("raster_backward_kernel");
 }
 

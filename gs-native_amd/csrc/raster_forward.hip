@@ -7,10 +7,13 @@
 namespace gsn {
 namespace {
 
+// Store = true additionally writes, per pixel, the final transmittance and the tile-relative
+// index + 1 of the last splat blended (0 when none): the state a back-to-front backward needs.
+template<bool Store>
 __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restrict__ ids,
     const int64_t* __restrict__ tile_offsets, const float2* __restrict__ means, const float4* __restrict__ conics,
     const float* __restrict__ colors, Camera cam, float* __restrict__ image, float bg_r, float bg_g, float bg_b,
-    int tile_count) {
+    int tile_count, float* __restrict__ final_transmittance, int* __restrict__ last_contributor) {
     const int tile = xcd_remap(blockIdx.x, tile_count);
     const int lane = threadIdx.x;
     const int x0 = (tile % cam.tile_count_x) * kTileSize;
@@ -19,6 +22,7 @@ __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restric
     const int py = y0 + (lane >> 4);
     const float fx = static_cast<float>(px);
     float fy[4], T[4], cr[4], cg[4], cb[4];
+    int last[4] = {0, 0, 0, 0};
     bool done[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -90,6 +94,9 @@ __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restric
                         cg[b] += g * w;
                         cb[b] += bl * w;
                         T[b] = next;
+                        if (Store) {
+                            last[b] = static_cast<int>(base - begin) + j + 1;
+                        }
                     }
                     done[b] = done[b] || finish;
                 }
@@ -100,10 +107,15 @@ __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restric
     for (int b = 0; b < 4; ++b) {
         int y = py + 4 * b;
         if (px < cam.width && y < cam.height) {
-            float* pixel = image + (static_cast<int64_t>(y) * cam.width + px) * 3;
+            int64_t index = static_cast<int64_t>(y) * cam.width + px;
+            float* pixel = image + index * 3;
             pixel[0] = cr[b] + T[b] * bg_r;
             pixel[1] = cg[b] + T[b] * bg_g;
             pixel[2] = cb[b] + T[b] * bg_b;
+            if (Store) {
+                final_transmittance[index] = T[b];
+                last_contributor[index] = last[b];
+            }
         }
     }
 }
@@ -111,10 +123,12 @@ __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restric
 } // namespace
 } // namespace gsn
 
-extern "C" int gsn_render_tiles(gsn_context* context, const gsn_camera* camera, const int* tile_gaussian_ids,
+namespace gsn {
+namespace {
+
+int render_tiles(gsn_context* context, const gsn_camera* camera, const int* tile_gaussian_ids,
     const int64_t* tile_offsets, const float* means, const float* conic_opacities, const float* colors,
-    const float* background, float* image) {
-    using namespace gsn;
+    const float* background, float* image, float* final_transmittance, int* last_contributor) {
     if (!context || !camera || !tile_offsets || !background || !image || camera->width <= 0 || camera->height <= 0 ||
         camera->tile_count_x <= 0 || camera->tile_count_y <= 0) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_render_tiles: invalid arguments");
@@ -126,8 +140,37 @@ extern "C" int gsn_render_tiles(gsn_context* context, const gsn_camera* camera, 
     Camera cam;
     memcpy(&cam, camera, sizeof(cam));
     StageScope scope(context, kStageRasterForward);
-    raster_forward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(tile_gaussian_ids, tile_offsets,
-        reinterpret_cast<const float2*>(means), reinterpret_cast<const float4*>(conic_opacities), colors, cam, image,
-        background[0], background[1], background[2], static_cast<int>(tiles));
+    if (final_transmittance) {
+        raster_forward_kernel<true><<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(tile_gaussian_ids,
+            tile_offsets, reinterpret_cast<const float2*>(means), reinterpret_cast<const float4*>(conic_opacities),
+            colors, cam, image, background[0], background[1], background[2], static_cast<int>(tiles),
+            final_transmittance, last_contributor);
+    }
+    else {
+        raster_forward_kernel<false><<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(tile_gaussian_ids,
+            tile_offsets, reinterpret_cast<const float2*>(means), reinterpret_cast<const float4*>(conic_opacities),
+            colors, cam, image, background[0], background[1], background[2], static_cast<int>(tiles), nullptr,
+            nullptr);
+    }
     return check_launch("raster_forward_kernel");
+}
+
+} // namespace
+} // namespace gsn
+
+extern "C" int gsn_render_tiles(gsn_context* context, const gsn_camera* camera, const int* tile_gaussian_ids,
+    const int64_t* tile_offsets, const float* means, const float* conic_opacities, const float* colors,
+    const float* background, float* image) {
+    return gsn::render_tiles(context, camera, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors,
+        background, image, nullptr, nullptr);
+}
+
+extern "C" int gsn_render_tiles_stored(gsn_context* context, const gsn_camera* camera, const int* tile_gaussian_ids,
+    const int64_t* tile_offsets, const float* means, const float* conic_opacities, const float* colors,
+    const float* background, float* image, float* final_transmittance, int32_t* last_contributor) {
+    if (!final_transmittance || !last_contributor) {
+        return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_render_tiles_stored: null per-pixel state");
+    }
+    return gsn::render_tiles(context, camera, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors,
+        background, image, final_transmittance, last_contributor);
 }

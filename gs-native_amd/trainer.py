@@ -34,21 +34,27 @@ FEATURE_STATE_ROWS = 6
 class OutcomeLog:
     """Training samples for the densification proposal: the features of every Gaussian at a
     density step paired with the outcome label of that step (gsn_density_outcomes), collected on
-    the host. Pruned Gaussians (label -1) are dropped."""
+    the host. Pruned Gaussians (label -1) are dropped; label-0 samples are kept with probability
+    keep_fraction and weighted 1 / keep_fraction, so the weighted loss stays unbiased."""
 
-    def __init__(self):
+    def __init__(self, keep_fraction: float = 1.0, seed: int = 0):
+        self.keep_fraction = keep_fraction
+        self.rng = np.random.default_rng(seed)
         self.features: list[np.ndarray] = []
         self.labels: list[np.ndarray] = []
+        self.weights: list[np.ndarray] = []
 
     def add(self, features: np.ndarray, labels: np.ndarray) -> None:
-        keep = labels >= 0
+        keep = labels > 0
+        keep |= (labels == 0) & (self.rng.random(labels.shape[0]) < self.keep_fraction)
         self.features.append(features[keep])
         self.labels.append(labels[keep].astype(np.int32))
+        self.weights.append(np.where(labels[keep] == 0, 1.0 / self.keep_fraction, 1.0).astype(np.float32))
 
-    def arrays(self) -> tuple[np.ndarray, np.ndarray]:
+    def arrays(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         if not self.features:
-            return np.zeros((0, FEATURES), np.float32), np.zeros(0, np.int32)
-        return np.concatenate(self.features), np.concatenate(self.labels)
+            return np.zeros((0, FEATURES), np.float32), np.zeros(0, np.int32), np.zeros(0, np.float32)
+        return np.concatenate(self.features), np.concatenate(self.labels), np.concatenate(self.weights)
 
 
 class Trainer:

@@ -178,6 +178,33 @@ int gsn_adam_update(gsn_context* context, int count, const float* gradients, flo
     float* first_moments, float* second_moments, float position_learning_rate,
     float first_correction, float second_correction);
 
+/* ---- stored-state rasteriser variant (no counterpart in the reference) -------------------
+ * For the peak-memory comparison with the recompute form above, which stores nothing per pixel.
+ * The forward additionally writes, per pixel, the final transmittance (float[H][W]) and the
+ * tile-relative index + 1 of the last splat blended (int[H][W], 0 if none): 8 bytes per pixel.
+ * The backward then walks each tile's records back to front from the last contributor,
+ * recovering T_k = T_{k+1} / (1 - alpha_k) and the colour behind each splat from T_N * background
+ * (the form of the original 3DGS rasteriser). Same outputs and partial layout as the recompute
+ * entries; background points to 3 host floats and must equal the forward's. */
+int gsn_render_tiles_stored(gsn_context* context, const gsn_camera* camera,
+    const int* tile_gaussian_ids, const int64_t* tile_offsets, const float* means,
+    const float* conic_opacities, const float* colors, const float* background, float* image,
+    float* final_transmittance, int32_t* last_contributor);
+int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_camera* camera, int count,
+    int sh_degree, const float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
+    int record_count, const int* record_offsets, const int* tile_rects, const float* means,
+    const float* conic_opacities, const float* colors, const float* final_transmittance,
+    const int32_t* last_contributor, const float* background, const float* image_gradients,
+    float* gradients, float* screen_gradients, float* statistics, float* absolute_statistics);
+int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gsn_camera* camera, int count,
+    int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
+    int record_count, const int* record_offsets, const int* tile_rects, const float* means,
+    const float* conic_opacities, const float* colors, const float* final_transmittance,
+    const int32_t* last_contributor, const float* background, const float* image_gradients,
+    float* first_moments, float* second_moments, float position_learning_rate,
+    float first_correction, float second_correction, float* statistics,
+    float* absolute_statistics);
+
 /* Gathers Adam moments by parent id (parent < 0 or >= previous_count -> zero).
  * parent_ids may be NULL (all zero). Replaces RemapGaussianHistoryKernel (solver.cu:625-649). */
 int gsn_remap_history(gsn_context* context, const int* parent_ids, int previous_count,

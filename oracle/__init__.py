@@ -242,6 +242,24 @@ def gradients_f64(params, camera, sh_degree, background, target, rects_from: dic
             **mapped}
 
 
+def raster_gradients_f64(camera, projected32: dict, background, target) -> dict:
+    """FP64 render, dL/dimage and replay backward on the FP32 projection's outputs (means, conics,
+    colours, depths, rectangles and visibility as the device computed them). Isolates the
+    rasteriser from the rounding of the projection, whose accuracy is checked on its own."""
+    visible = (projected32["record_counts"][:-1] > 0).astype(np.uint8)
+    projected = {
+        "means": projected32["means"].astype(np.float64), "conics": projected32["conic_opacities"].astype(np.float64),
+        "colors": projected32["colors"].astype(np.float64), "depths": projected32["depths"].astype(np.float64),
+        "tile_rects": np.ascontiguousarray(projected32["tile_rects"], np.int32), "visible": visible,
+    }
+    image, ambiguous = render_f64(camera, projected, background)
+    image_grad = image_gradients_f64(image, target)
+    difference = np.abs(image - np.asarray(target, np.float64) / 255.0)
+    loss_ambiguous = np.any((difference <= 1e-5 * np.abs(image)) & (image != 0.0), axis=2)
+    back = backward_f64(camera, projected, image, image_grad, loss_ambiguous)
+    return {"image": image, "ambiguous": ambiguous, "image_grad": image_grad, **back}
+
+
 def adam_f32(gradients, params, first, second, position_lr, first_correction, second_correction):
     gradients, params, first, second = (_f32(a).copy() for a in (gradients, params, first, second))
     n = params.shape[1]
