@@ -338,8 +338,11 @@ class Projection:
 
 
 def render(context: Context, params: DeviceBuffer, count: int, camera: Camera, background, image: DeviceBuffer,
-           sh_degree: int, projection: Projection | None = None, keep_keys: bool = False) -> Projection:
-    """Projection -> record scan -> tile lists -> raster forward (Renderer::Render)."""
+           sh_degree: int, projection: Projection | None = None, keep_keys: bool = False,
+           stored_state: tuple | None = None) -> Projection:
+    """Projection -> record scan -> tile lists -> raster forward (Renderer::Render).
+    stored_state = (final_transmittance, last_contributor) selects the forward that also writes the
+    per-pixel state of the stored-state backward."""
     if projection is None or projection.count != count:
         projection = Projection(count, camera)
     projection.camera = camera
@@ -362,8 +365,14 @@ def render(context: Context, params: DeviceBuffer, count: int, camera: Camera, b
          _p(projection.record_offsets), records.value, _p(projection.tile_gaussian_ids), _p(projection.tile_offsets),
          _p(keys))
     bg = (ctypes.c_float * 3)(*[float(v) for v in background])
-    call("gsn_render_tiles", h, ctypes.byref(camera), _p(projection.tile_gaussian_ids), _p(projection.tile_offsets),
-         _p(projection.means), _p(projection.conic_opacities), _p(projection.colors), bg, _p(image))
+    if stored_state is not None:
+        call("gsn_render_tiles_stored", h, ctypes.byref(camera), _p(projection.tile_gaussian_ids),
+             _p(projection.tile_offsets), _p(projection.means), _p(projection.conic_opacities), _p(projection.colors),
+             bg, _p(image), _p(stored_state[0]), _p(stored_state[1]))
+    else:
+        call("gsn_render_tiles", h, ctypes.byref(camera), _p(projection.tile_gaussian_ids),
+             _p(projection.tile_offsets), _p(projection.means), _p(projection.conic_opacities), _p(projection.colors),
+             bg, _p(image))
     return projection
 
 

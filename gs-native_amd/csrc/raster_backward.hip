@@ -18,8 +18,6 @@ namespace gsn {
 
 namespace {
 
-typedef float float4_vector __attribute__((ext_vector_type(4)));
-
 // Everything the backward needs of one Gaussian in 48 bytes, so that each record costs one
 // gather instead of five (means, conics, colours, rectangles, record offsets):
 //   a = (mean x, mean y, conic A, conic B), b = (conic C, opacity, red, green),
@@ -53,11 +51,10 @@ __device__ __forceinline__ int64_t partial_slot(float4 c, int tile_x, int tile_y
     return static_cast<int64_t>(__float_as_int(c.y)) + (tile_y - (origin >> 16)) * width + (tile_x - (origin & 0xffff));
 }
 
-// Partials are written once and read once by the per-Gaussian pass: non-temporal stores keep
-// them from displacing the Gaussian records that later tiles gather.
+// Plain stores: non-temporal stores of these scattered 16-byte pieces measured 2.3x the write
+// traffic and twice the kernel time (each piece left the cache on its own).
 __device__ __forceinline__ void store_partial(float* destination, const float4& value) {
-    __builtin_nontemporal_store(float4_vector{value.x, value.y, value.z, value.w},
-        reinterpret_cast<float4_vector*>(destination));
+    *reinterpret_cast<float4*>(destination) = value;
 }
 
 __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restrict__ ids,

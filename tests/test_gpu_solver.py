@@ -58,10 +58,12 @@ def test_screen_gradients_match_fp64_backward(gsn, context, oracle, scenes, sh_d
     run = GpuRun(gsn, context, params, cam, (0.0, 0.0, 0.0), sh_degree)
     got = run.gradients(target)
     f32 = oracle.project_f32(params, cam, sh_degree)
-    # The FP64 replay runs on the FP32 projection (bit-identical to the device's), so that the
-    # comparison measures the rasteriser: the projected means alone carry an absolute FP32 error of
-    # ~|mean| * 6e-8 px, which moves alpha by |q| * that for small splats.
-    ref = oracle.raster_gradients_f64(cam, f32, (0.0, 0.0, 0.0), target)
+    # The FP64 replay runs on the FP32 projection (bit-identical to the device's) and on the
+    # device's dL/dimage, so that the comparison measures the rasteriser backward alone: the
+    # projected means carry an absolute FP32 error of ~|mean| * 6e-8 px, which moves alpha by |q|
+    # times that for small splats, and the SSIM gradient's local variances cancel in flat regions
+    # (both inputs are checked against FP64 by their own tests).
+    ref = oracle.raster_gradients_f64(cam, f32, (0.0, 0.0, 0.0), target, image_grad=got["image_grad"])
     # Gaussians that contribute to a pixel where some threshold decision (alpha vs 1/255, the
     # 0.99 cap, T vs 1e-4) lies within 1e-4 relative of its threshold may legitimately differ
     # between FP32 and FP64; in this dense scene every such pixel has ~40 contributors.
