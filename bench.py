@@ -59,14 +59,15 @@ def algorithmic_bytes(n: int, v: float, r: float, p: int, t: int, k: int = 16) -
     }
 
 
-def measured_traffic(stage: str):
-    """HBM bytes per launch of a stage's kernel as measured by rocprofv3 FETCH_SIZE + WRITE_SIZE on
-    this workload (committed under profiles/, scaled by a plain copy of known size), or None."""
+def profiled_value(key: str, stage: str):
+    """A per-kernel figure recorded by a committed rocprofv3 summary of this workload under
+    profiles/ ("<key> <stage> = <value>" lines), or None: traffic_bytes_per_launch (FETCH_SIZE +
+    WRITE_SIZE scaled by a plain copy of known size) and valu_share (SQ_INSTS_VALU issue share)."""
     for path in sorted((REPO / "profiles").glob("*.md")):
         for line in path.read_text().splitlines():
-            match = re.match(r"^traffic_bytes_per_launch\s+(\w+)\s*=\s*([0-9.eE+]+)", line.strip())
-            if match and match.group(1) == stage:
-                return float(match.group(2))
+            match = re.match(r"^(\w+)\s+(\w+)\s*=\s*([0-9.eE+-]+)$", line.strip())
+            if match and match.group(1) == key and match.group(2) == stage:
+                return float(match.group(3))
     return None
 
 
@@ -249,7 +250,8 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
-    result["roofline"]["traffic"] = measured_traffic(dominant)
+    result["roofline"]["traffic"] = profiled_value("traffic_bytes_per_launch", dominant)
+    result["roofline"]["valu_share"] = profiled_value("valu_share", dominant)
     if args.calibration_copy_mib > 0:
         size = args.calibration_copy_mib << 20
         source = gsn.DeviceBuffer((size,), np.uint8)

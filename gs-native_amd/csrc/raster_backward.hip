@@ -90,6 +90,18 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
     }
     const int64_t begin = tile_offsets[tile];
     const int64_t end = tile_offsets[tile + 1];
+    // Software pipeline over record batches: while batch k is processed, the splat records of
+    // batch k + 1 and the ids of batch k + 2 are in flight, so the two dependent gathers of a
+    // batch (id, then its 48-byte record) are hidden behind a batch of compute.
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 current_a = zero, current_b = zero, current_c = zero;
+    if (begin + lane < end) {
+        const PackedSplat* first = splats + ids[begin + lane];
+        current_a = first->a;
+        current_b = first->b;
+        current_c = first->c;
+    }
+    int following_id = begin + 64 + lane < end ? ids[begin + 64 + lane] : -1;
     int64_t base = begin;
     for (; base < end; base += 64) {
         int active = 0;
@@ -102,6 +114,14 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
         if (active == 0) {
             break;
         }
+        const float4 splat_a = current_a, splat_b = current_b, splat_c = current_c;
+        if (following_id >= 0) {
+            const PackedSplat* next = splats + following_id;
+            current_a = next->a;
+            current_b = next->b;
+            current_c = next->c;
+        }
+        following_id = base + 128 + lane < end ? ids[base + 128 + lane] : -1;
         const int64_t record = base + lane;
         const bool valid = record < end;
         float mx = 0.0f, my = 0.0f, sr = 0.0f, sg = 0.0f, sb = 0.0f;
@@ -109,15 +129,14 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
         int mask = 0;
         int64_t slot = 0;
         if (valid) {
-            const PackedSplat splat = splats[ids[record]];
-            mx = splat.a.x;
-            my = splat.a.y;
-            conic = make_float4(splat.a.z, splat.a.w, splat.b.x, splat.b.y);
-            sr = splat.b.z;
-            sg = splat.b.w;
-            sb = splat.c.x;
+            mx = splat_a.x;
+            my = splat_a.y;
+            conic = make_float4(splat_a.z, splat_a.w, splat_b.x, splat_b.y);
+            sr = splat_b.z;
+            sg = splat_b.w;
+            sb = splat_c.x;
             mask = band_mask(mx, my, conic, x0, y0);
-            slot = partial_slot(splat.c, tile_x, tile_y);
+            slot = partial_slot(splat_c, tile_x, tile_y);
         }
 #pragma unroll
         for (int k = 0; k < kPartialFloats; ++k) {
