@@ -284,7 +284,7 @@ __device__ __forceinline__ void adam_entry(float* params, float* first, float* s
     params[index] = p;
 }
 
-constexpr int kAdamGroup = 8;
+constexpr int kAdamGroup = 16;
 
 constexpr int kGradientBlock = 256;
 constexpr int kGradientChunk = 256;  // partial records staged per LDS round (12 KiB)

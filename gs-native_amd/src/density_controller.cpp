@@ -22,6 +22,9 @@ DensityController::~DensityController() {
     DeviceFree(observation_counts_);
     DeviceFree(density_counts_);
     DeviceFree(density_offsets_);
+    DeviceFree(feature_state_);
+    DeviceFree(features_);
+    DeviceFree(logits_);
     DeviceFree(control_);
 }
 
@@ -139,6 +142,48 @@ bool DensityController::AccumulateStatistics(const float* statistics, const floa
         "gsn_density_accumulate");
 }
 
+// Feature buffers sized for `count` Gaussians (stride feature_capacity_). They only grow right
+// after a density step, when the state has just been cleared, so nothing is carried over.
+bool DensityController::ReserveFeatures(int count) {
+    if (count <= feature_capacity_) {
+        return true;
+    }
+    int capacity = std::max(count + count / 4, 1);
+    float* state = nullptr;
+    float* features = nullptr;
+    float* logits = nullptr;
+    size_t n = static_cast<size_t>(capacity);
+    if (!DeviceAllocateCount(&state, 6 * n) || !DeviceAllocateCount(&features, 16 * n) ||
+        !DeviceAllocateCount(&logits, 3 * n) ||
+        !Check(gsn_density_features_clear(DefaultContext(), state, capacity), "gsn_density_features_clear")) {
+        DeviceFree(state);
+        DeviceFree(features);
+        DeviceFree(logits);
+        return false;
+    }
+    DeviceFree(feature_state_);
+    DeviceFree(features_);
+    DeviceFree(logits_);
+    feature_state_ = state;
+    features_ = features;
+    logits_ = logits;
+    feature_capacity_ = capacity;
+    return true;
+}
+
+bool DensityController::AccumulateFeatures(const float* statistics, const float* absolute_statistics,
+    const float* depths, const float* conic_opacities, int gaussian_count) {
+    if (!proposal_ || gaussian_count == 0) {
+        return true;
+    }
+    if (!control_ || gaussian_count < 0 || !ReserveFeatures(gaussian_count)) {
+        return false;
+    }
+    return Check(gsn_density_features_accumulate(DefaultContext(), statistics, absolute_statistics, depths,
+                     conic_opacities, gaussian_count, feature_state_, feature_capacity_),
+        "gsn_density_features_accumulate");
+}
+
 bool DensityController::ResetOpacity(float* parameters, int stride, float max_opacity) {
     if (!control_ || stride != parameter_stride_ || (!parameters && stride > 0) || !std::isfinite(max_opacity) ||
         max_opacity <= 0.0f || max_opacity >= 1.0f) {
@@ -167,11 +212,28 @@ bool DensityController::ControlDensity(GaussianBuffers& gaussians, float scene_e
         return false;
     }
     int totals[4] = {0, 0, 0, 0};
-    if (!Check(gsn_density_count(context, parameters, count, gradient_sums_, absolute_gradient_sums_,
-                   observation_counts_, gradient_threshold, percent_dense * scene_extent, min_opacity,
-                   0.1f * scene_extent, max_screen_size > 0, absolute_gradient_threshold, density_counts_,
-                   density_offsets_, totals),
-            "gsn_density_count")) {
+    float split_threshold = percent_dense * scene_extent;
+    bool proposed = proposal_ && proposal_->IsLoaded() && count > 0 && gradient_threshold > 0.0f &&
+        split_threshold > 0.0f && scene_extent > 0.0f;
+    if (proposed) {
+        // Learned proposal: features -> logits -> decisions; pruning stays the rule's.
+        if (!ReserveFeatures(count) ||
+            !Check(gsn_density_features(context, parameters, count, gradient_sums_, absolute_gradient_sums_,
+                       observation_counts_, feature_state_, feature_capacity_, gradient_threshold, split_threshold,
+                       scene_extent, features_),
+                "gsn_density_features") ||
+            !proposal_->Predict(features_, count, logits_) ||
+            !Check(gsn_density_count_proposed(context, parameters, count, gradient_sums_, absolute_gradient_sums_,
+                       observation_counts_, logits_, split_threshold, min_opacity, 0.1f * scene_extent,
+                       max_screen_size > 0, density_counts_, density_offsets_, totals),
+                "gsn_density_count_proposed")) {
+            return false;
+        }
+    }
+    else if (!Check(gsn_density_count(context, parameters, count, gradient_sums_, absolute_gradient_sums_,
+                        observation_counts_, gradient_threshold, split_threshold, min_opacity, 0.1f * scene_extent,
+                        max_screen_size > 0, absolute_gradient_threshold, density_counts_, density_offsets_, totals),
+                 "gsn_density_count")) {
         return false;
     }
     int required = totals[0] + totals[1] + 2 * totals[3];
@@ -223,7 +285,9 @@ bool DensityController::ControlDensity(GaussianBuffers& gaussians, float scene_e
                         child_flags_),
                   "gsn_density_write") &&
         Check(gsn_density_clear(context, gradient_sums_, absolute_gradient_sums_, observation_counts_, count),
-            "gsn_density_clear");
+            "gsn_density_clear") &&
+        (!feature_state_ ||
+            Check(gsn_density_features_clear(context, feature_state_, feature_capacity_), "gsn_density_features_clear"));
     if (!ok) {
         return false;
     }

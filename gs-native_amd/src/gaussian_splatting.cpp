@@ -181,7 +181,10 @@ bool GaussianSplatting::Train(int iterations) {
         ++camera_cursor_;
         if (completed_ < 15000) {
             if (!density_.AccumulateStatistics(solver_.GetProjectionStatistics(),
-                    solver_.GetAbsoluteProjectionStatistics(), gaussians_.gaussian_count_)) {
+                    solver_.GetAbsoluteProjectionStatistics(), gaussians_.gaussian_count_) ||
+                !density_.AccumulateFeatures(solver_.GetProjectionStatistics(),
+                    solver_.GetAbsoluteProjectionStatistics(), renderer_.GetDepths(),
+                    reinterpret_cast<const float*>(renderer_.GetConicOpacities()), gaussians_.gaussian_count_)) {
                 return false;
             }
             if (completed_ > 500 && completed_ % 100 == 0 && !ControlDensity()) {

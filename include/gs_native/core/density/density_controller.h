@@ -4,6 +4,7 @@
 // reference does. GetGaussianCount/GetStatus/GetParentIds return device pointers.
 #pragma once
 
+#include "core/density/proposal_network.h"
 #include "core/types/buffers.h"
 
 #include <cstddef>
@@ -69,8 +70,29 @@ public:
         return child_flags_;
     }
 
+    // Learned densification (additions, no counterpart in the reference). With a proposal set,
+    // ControlDensity builds the 16 features of every Gaussian, runs the network and lets its
+    // logits choose keep / clone / split (gsn_density_count_proposed); pruning, compaction,
+    // jitter and the buffer swap are unchanged. nullptr restores the gradient-threshold rule.
+    // The features need AccumulateFeatures after every AccumulateStatistics.
+    void SetProposal(const ProposalNetwork* network) {
+        proposal_ = network;
+    }
+
+    const ProposalNetwork* GetProposal() const {
+        return proposal_;
+    }
+
+    bool AccumulateFeatures(
+        const float* statistics,
+        const float* absolute_statistics,
+        const float* depths,
+        const float* conic_opacities,
+        int gaussian_count);
+
 private:
     bool Reserve(int capacity);
+    bool ReserveFeatures(int count);
     bool WriteControl(int count, const int* totals, int previous_count);
 
     float* spare_gaussian_parameters_ = nullptr;
@@ -85,6 +107,11 @@ private:
     int* density_counts_ = nullptr;
     int* density_offsets_ = nullptr;
     int* control_ = nullptr;
+    const ProposalNetwork* proposal_ = nullptr;
+    float* feature_state_ = nullptr;  // [6][capacity_]
+    float* features_ = nullptr;       // [capacity_][16]
+    float* logits_ = nullptr;         // [capacity_][3]
+    int feature_capacity_ = 0;
     int capacity_ = 0;
     int parameter_stride_ = 0;
     unsigned long long seed_ = 0;

@@ -34,6 +34,12 @@ public:
         return gaussians_.gaussian_count_;
     }
 
+    // Learned densification (addition): with a loaded network, density control takes keep /
+    // clone / split from it (DensityController::SetProposal); nullptr restores the plain rule.
+    void SetDensityProposal(const ProposalNetwork* network) {
+        density_.SetProposal(network);
+    }
+
 private:
     bool InitializeGaussians();
     bool ControlDensity();

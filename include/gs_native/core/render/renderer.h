@@ -56,6 +56,12 @@ public:
         return tile_rects_;
     }
 
+    // Camera-space depth per Gaussian of the last Render (addition: the learned densification
+    // features read it; the reference keeps depths internal).
+    const float* GetDepths() const {
+        return depths_;
+    }
+
     int GetRecordCount() const {
         return record_count_;
     }

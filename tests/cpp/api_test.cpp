@@ -318,6 +318,55 @@ void DensityControllerPrunesAndSwaps() {
     EXPECT(everything.gaussian_count_ == 0 && everything.gaussians_ == nullptr);
 }
 
+// Learned densification hook (no reference counterpart): a proposal that always answers keep adds
+// nothing, one that always answers clone clones every survivor; pruning stays the rule's.
+void ProposalNetworkDrivesDensityControl() {
+    int n = ProposalNetwork::ParameterCount();
+    EXPECT(n == 5443);
+    std::vector<float> keep(static_cast<size_t>(n), 0.0f);
+    keep[n - 3] = 1.0f;
+    ProposalNetwork keep_all;
+    EXPECT(keep_all.Load(keep));
+    EXPECT(keep_all.Parameters() == keep);
+    DeviceVector<float> features(std::vector<float>(5 * 16, 0.5f));
+    DeviceVector<float> logits(5 * 3);
+    EXPECT(keep_all.Predict(features.data, 5, logits.data));
+    auto values = logits.Download();
+    for (int i = 0; i < 5; ++i) {
+        EXPECT(values[3 * i] == 1.0f && values[3 * i + 1] == 0.0f && values[3 * i + 2] == 0.0f);
+    }
+    std::vector<float> clone(static_cast<size_t>(n), 0.0f);
+    clone[n - 2] = 1.0f;
+    ProposalNetwork clone_all;
+    EXPECT(clone_all.Load(clone));
+
+    for (int variant = 0; variant < 2; ++variant) {
+        auto gaussians = MakeGaussians(3);
+        gaussians[1].opacity_logit_ = std::log(0.001f / 0.999f);
+        DeviceVector<float> original(PackGaussians(gaussians));
+        DensityController controller;
+        EXPECT(controller.Initialize(3, 3));
+        controller.SetProposal(variant == 0 ? &keep_all : &clone_all);
+        // every Gaussian observed once with a screen gradient far above the threshold
+        DeviceVector<float> statistics(std::vector<float>{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f});
+        DeviceVector<float> absolute(std::vector<float>{1.0f, 1.0f, 1.0f, 0.0f, 0.0f, 0.0f});
+        EXPECT(controller.AccumulateStatistics(statistics.data, absolute.data, 3));
+        float* copy = nullptr;
+        EXPECT(hipMalloc(reinterpret_cast<void**>(&copy), original.count * sizeof(float)) == hipSuccess);
+        EXPECT(hipMemcpy(copy, original.data, original.count * sizeof(float), hipMemcpyDeviceToDevice) == hipSuccess);
+        GaussianBuffers buffers{copy, 3};
+        int changes[2] = {};
+        EXPECT(controller.ControlDensity(buffers, 1.0f, 0.15f, 0.01f, 0.005f, 0.0f, 42, changes));
+        if (variant == 0) {
+            EXPECT(changes[0] == 0 && changes[1] == 1 && buffers.gaussian_count_ == 2);
+        }
+        else {
+            EXPECT(changes[0] == 2 && changes[1] == 1 && buffers.gaussian_count_ == 4);
+        }
+        (void)hipFree(buffers.gaussians_);
+    }
+}
+
 // solver_test.cu:320-336: opacity history reset -> first-step magnitude at step 2.
 void SolverOpacityHistoryReset() {
     CameraParams camera = MakeCamera(9, 7);
@@ -368,6 +417,7 @@ int main(int argc, char** argv) {
         {"DensityControllerPrunesAndSwaps", DensityControllerPrunesAndSwaps},
         {"SolverOpacityHistoryReset", SolverOpacityHistoryReset},
         {"KnnPreservesOrder", KnnPreservesOrder},
+        {"ProposalNetworkDrivesDensityControl", ProposalNetworkDrivesDensityControl},
     };
     int failed = 0;
     for (auto& [name, test] : tests) {
