@@ -287,42 +287,63 @@ __device__ __forceinline__ void adam_entry(float* params, float* first, float* s
 constexpr int kAdamGroup = 16;
 
 constexpr int kGradientBlock = 256;
-constexpr int kGradientChunk = 256;  // partial records staged per LDS round (12 KiB)
 
 template<int Degree, bool Fused>
 __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const float* __restrict__ partials,
     const int* __restrict__ record_offsets, int count, int record_count, Camera cam, float* params, float* gradients, float* screen,
     float* statistics, float* absolute_statistics, float* first, float* second, float position_learning_rate,
     float first_correction, float second_correction) {
-    // The block's Gaussians own one contiguous run of partial records. It is staged through LDS
-    // in chunks with coalesced 16-byte loads; each thread then sums its own records in order.
-    __shared__ float4 s_records[kGradientChunk * 3];
-    const int first_id = blockIdx.x * kGradientBlock;
-    const int id = first_id + threadIdx.x;
+    // Each wave's 64 Gaussians own one contiguous run of partial records. The wave streams it
+    // through its own LDS slice in 64-record chunks (coalesced 16-byte loads, the next chunk held
+    // in registers while the current one is summed); each lane then sums its own records in
+    // order. Waves never wait for each other (no block barrier).
+    __shared__ float4 s_records[kGradientBlock / 64][64 * 3];
+    const int lane = threadIdx.x & 63;
+    float4* slice = s_records[threadIdx.x >> 6];
+    const int id = blockIdx.x * kGradientBlock + threadIdx.x;
+    const int wave_first = id - lane;
     const int64_t n = count;
-    const int block_begin = min(record_offsets[first_id], record_count);
-    const int block_end = min(record_offsets[min(first_id + kGradientBlock, count)], record_count);
     int begin = 0, end = 0;
     if (id < count) {
         begin = min(record_offsets[id], record_count);
         end = min(record_offsets[id + 1], record_count);
     }
+    const int wave_begin = wave_first < count ? min(record_offsets[wave_first], record_count) : 0;
+    const int wave_end = wave_first < count ? min(record_offsets[min(wave_first + 64, count)], record_count) : 0;
     float in[11];
 #pragma unroll
     for (int k = 0; k < 11; ++k) {
         in[k] = 0.0f;
     }
     const float4* records = reinterpret_cast<const float4*>(partials);
-    for (int chunk = block_begin; chunk < block_end; chunk += kGradientChunk) {
-        const int chunk_end = min(chunk + kGradientChunk, block_end);
-        for (int i = threadIdx.x; i < (chunk_end - chunk) * 3; i += kGradientBlock) {
-            s_records[i] = records[static_cast<int64_t>(chunk) * 3 + i];
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int64_t limit = static_cast<int64_t>(wave_end) * 3;
+    float4 staged[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        int64_t index = static_cast<int64_t>(wave_begin) * 3 + r * 64 + lane;
+        staged[r] = zero;
+        if (index < limit) {
+            staged[r] = records[index];
         }
-        __syncthreads();
-        const int last = min(end, chunk_end);
+    }
+    for (int chunk = wave_begin; chunk < wave_end; chunk += 64) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            slice[r * 64 + lane] = staged[r];
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int next = chunk + 64;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            int64_t index = static_cast<int64_t>(next) * 3 + r * 64 + lane;
+            if (index < limit) {
+                staged[r] = records[index];
+            }
+        }
+        const int last = min(end, next);
         for (int p = max(begin, chunk); p < last; ++p) {
-            const float4 a = s_records[(p - chunk) * 3], b = s_records[(p - chunk) * 3 + 1],
-                         c = s_records[(p - chunk) * 3 + 2];
+            const float4 a = slice[(p - chunk) * 3], b = slice[(p - chunk) * 3 + 1], c = slice[(p - chunk) * 3 + 2];
             in[0] += a.x;
             in[1] += a.y;
             in[2] += a.z;
@@ -335,7 +356,7 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
             in[9] += c.y;
             in[10] += c.z;
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
     }
     if (id >= count) {
         return;

@@ -160,6 +160,9 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
 #pragma unroll
             for (int k = 0; k < 12; ++k) {
                 acc[k] = 0.0f;
+                // Opaque to the optimiser: otherwise it folds the zero into the first band's
+                // update and rebuilds twelve zeros on every path that skips a band.
+                asm volatile("" : "+v"(acc[k]));
             }
             bool contributed = false;
 #pragma unroll
@@ -174,9 +177,12 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
                     bool finish = take && next < kTransmittanceMin;
                     bool blend = take && !finish;
                     done[b] = done[b] || finish;
-                    if (blend) {
+                    // Uniform branch on "some lane blends"; inside, lanes that do not blend add
+                    // exact zeros (selects, not divergent ifs), which keeps the accumulators in
+                    // place instead of merging per-lane control paths.
+                    if (__ballot(blend)) {
                         contributed = true;
-                        float weight = alpha * T[b];
+                        float weight = blend ? alpha * T[b] : 0.0f;
                         float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
                         float alpha_gradient = 0.0f;
 #pragma unroll
@@ -184,30 +190,28 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
                             alpha_gradient += grad[b][c] * (prefix[b][c] + T[b] * col[c] - final_color[b][c]) * inv;
                             acc[6 + c] += weight * grad[b][c];
                         }
-                        if (raw_alpha < kAlphaMax) {
-                            float common = alpha_gradient * raw_alpha;
-                            float qx = ca * dx + cbv * dy;
-                            float qy = cbv * dx + cc * dy;
-                            float gx = -common * qx;
-                            float gy = -common * qy;
-                            acc[0] += gx;
-                            acc[1] += gy;
-                            acc[2] += 0.5f * common * qx * qx;
-                            acc[3] += common * qx * qy;
-                            acc[4] += 0.5f * common * qy * qy;
-                            acc[5] += common * (1.0f - op);
-                            acc[9] += fabsf(gx);
-                            acc[10] += fabsf(gy);
-                        }
+                        float common = blend && raw_alpha < kAlphaMax ? alpha_gradient * raw_alpha : 0.0f;
+                        float qx = ca * dx + cbv * dy;
+                        float qy = cbv * dx + cc * dy;
+                        float gx = -common * qx;
+                        float gy = -common * qy;
+                        acc[0] += gx;
+                        acc[1] += gy;
+                        acc[2] += 0.5f * common * qx * qx;
+                        acc[3] += common * qx * qy;
+                        acc[4] += 0.5f * common * qy * qy;
+                        acc[5] += common * (1.0f - op);
+                        acc[9] += fabsf(gx);
+                        acc[10] += fabsf(gy);
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
                             prefix[b][c] += col[c] * weight;
                         }
-                        T[b] = next;
+                        T[b] = blend ? next : T[b];
                     }
                 }
             }
-            if (__ballot(contributed)) {
+            if (contributed) {
                 float sums[3];
                 wave_sum12(acc, sums);
                 if ((lane & 15) == 0) {
