@@ -27,12 +27,19 @@ __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(const uin
     __syncthreads();
     int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
     uint32_t mask = static_cast<uint32_t>(bins - 1);
-#pragma unroll 4
+    uint32_t loaded[kSortItems];
+#pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
         int64_t index = base + i * kSortThreads + t;
+        loaded[i] = 0u;
         if (index < n) {
-            uint32_t digit = (keys[index] >> shift) & mask;
-            atomicAdd(&counts[digit], 1);
+            loaded[i] = keys[index];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        if (base + i * kSortThreads + t < n) {
+            atomicAdd(&counts[(loaded[i] >> shift) & mask], 1);
         }
     }
     __syncthreads();
@@ -67,12 +74,22 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
     uint32_t key_reg[kSortItems];
     uint32_t value_reg[kSortItems];
     int rank_reg[kSortItems];
+    // All loads first (32 per lane in flight), then the ranking, which serialises on LDS counters.
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        int local = wave * kWaveChunk + j * 64 + lane;
+        key_reg[j] = 0xffffffffu;
+        value_reg[j] = 0u;
+        if (local < valid) {
+            key_reg[j] = keys_in[tile_base + local];
+            value_reg[j] = values_in[tile_base + local];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         int local = wave * kWaveChunk + j * 64 + lane;
         bool in_range = local < valid;
-        uint32_t key = in_range ? keys_in[tile_base + local] : 0xffffffffu;
-        uint32_t value = in_range ? values_in[tile_base + local] : 0u;
+        uint32_t key = key_reg[j];
         // Out-of-range slots take the largest digit so that they rank after every valid element.
         uint32_t digit = in_range ? ((key >> shift) & mask) : mask;
         uint64_t peers = ~0ull;
@@ -88,8 +105,6 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
         }
         // Keeps the next round's counter reads after this round's leader writes.
         __builtin_amdgcn_wave_barrier();
-        key_reg[j] = key;
-        value_reg[j] = value;
     }
     __syncthreads();
 
