@@ -15,9 +15,58 @@ namespace {
 
 struct Mapped {
     float grad[10];
-    float basis[16];
+    float x, y, z;  // unit view direction (SH basis argument)
     bool color_ok[3];
 };
+
+// Real SH basis function k (degree <= 3) at the unit direction (x, y, z); constants as the
+// reference's ComputeEquationSh (solver.cu:223-323). k is a compile-time constant after unrolling.
+__device__ __forceinline__ float sh_basis_value(int k, float x, float y, float z) {
+    switch (k) {
+    case 0: return kShC0;
+    case 1: return -kShC1 * y;
+    case 2: return kShC1 * z;
+    case 3: return -kShC1 * x;
+    case 4: return kShC2_0 * x * y;
+    case 5: return -kShC2_0 * y * z;
+    case 6: return kShC2_2 * (2.0f * z * z - x * x - y * y);
+    case 7: return -kShC2_0 * x * z;
+    case 8: return kShC2_4 * (x * x - y * y);
+    case 9: return kShC3_0 * y * (3.0f * x * x - y * y);
+    case 10: return kShC3_1 * x * y * z;
+    case 11: return kShC3_2 * y * (4.0f * z * z - x * x - y * y);
+    case 12: return kShC3_3 * z * (2.0f * z * z - 3.0f * x * x - 3.0f * y * y);
+    case 13: return kShC3_2 * x * (4.0f * z * z - x * x - y * y);
+    case 14: return kShC3_5 * z * (x * x - y * y);
+    default: return kShC3_0 * x * (x * x - 3.0f * y * y);
+    }
+}
+
+// Partial derivatives of sh_basis_value(k, .) with respect to x, y and z (before projecting out
+// the radial part).
+__device__ __forceinline__ void sh_basis_derivative(int k, float x, float y, float z, float (&d)[3]) {
+    constexpr float c = kShC2_0, dd = kShC2_2, e = kShC2_4;
+    constexpr float a3 = kShC3_0, b3 = kShC3_1, c3 = kShC3_2, d3 = kShC3_3, e3 = kShC3_5;
+    d[0] = d[1] = d[2] = 0.0f;
+    switch (k) {
+    case 1: d[1] = -kShC1; break;
+    case 2: d[2] = kShC1; break;
+    case 3: d[0] = -kShC1; break;
+    case 4: d[0] = c * y; d[1] = c * x; break;
+    case 5: d[1] = -c * z; d[2] = -c * y; break;
+    case 6: d[0] = -2.0f * dd * x; d[1] = -2.0f * dd * y; d[2] = 4.0f * dd * z; break;
+    case 7: d[0] = -c * z; d[2] = -c * x; break;
+    case 8: d[0] = 2.0f * e * x; d[1] = -2.0f * e * y; break;
+    case 9: d[0] = 6.0f * a3 * x * y; d[1] = 3.0f * a3 * (x * x - y * y); break;
+    case 10: d[0] = b3 * y * z; d[1] = b3 * x * z; d[2] = b3 * x * y; break;
+    case 11: d[0] = -2.0f * c3 * x * y; d[1] = c3 * (4.0f * z * z - x * x - 3.0f * y * y); d[2] = 8.0f * c3 * y * z; break;
+    case 12: d[0] = -6.0f * d3 * x * z; d[1] = -6.0f * d3 * y * z; d[2] = d3 * (6.0f * z * z - 3.0f * x * x - 3.0f * y * y); break;
+    case 13: d[0] = c3 * (4.0f * z * z - 3.0f * x * x - y * y); d[1] = -2.0f * c3 * x * y; d[2] = 8.0f * c3 * x * z; break;
+    case 14: d[0] = 2.0f * e3 * x * z; d[1] = -2.0f * e3 * y * z; d[2] = e3 * (x * x - y * y); break;
+    case 15: d[0] = 3.0f * a3 * (x * x - y * y); d[1] = -6.0f * a3 * x * y; break;
+    default: break;
+    }
+}
 
 // Chain rule from the 9 screen-space gradients to position, log-scale, axis-angle and opacity
 // gradients; SH gradients are in[6 + c] * basis[k] for colour channels with value >= 0.
@@ -83,7 +132,8 @@ __device__ __forceinline__ void map_gaussian(const float* __restrict__ params, c
         }
     }
 
-    // SH colour (unclamped) and its derivative with respect to the world position.
+    // SH colour (unclamped) and its derivative with respect to the world position, one basis
+    // function at a time so that no basis or derivative table stays live.
     float direction[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -98,91 +148,34 @@ __device__ __forceinline__ void map_gaussian(const float* __restrict__ params, c
     float x = direction[0] * inverse_length;
     float y = direction[1] * inverse_length;
     float z = direction[2] * inverse_length;
-    float d[48];
+    float value[3] = {0.0f, 0.0f, 0.0f};
+    float derivative[3][3] = {};
 #pragma unroll
-    for (int k = 0; k < 48; ++k) {
-        d[k] = 0.0f;
-    }
+    for (int j = 0; j < s; ++j) {
+        float basis = sh_basis_value(j, x, y, z);
+        float d[3];
+        sh_basis_derivative(j, x, y, z, d);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        out.basis[k] = 0.0f;
+        for (int channel = 0; channel < 3; ++channel) {
+            float coefficient = params[(kShIndex + channel * 16 + j) * n + id];
+            value[channel] += coefficient * basis;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                derivative[channel][a] += coefficient * d[a];
+            }
+        }
     }
-    out.basis[0] = kShC0;
-    if (Degree >= 1) {
-        out.basis[1] = -kShC1 * y;
-        out.basis[2] = kShC1 * z;
-        out.basis[3] = -kShC1 * x;
-        d[4] = -kShC1;
-        d[8] = kShC1;
-        d[9] = -kShC1;
-    }
-    if (Degree >= 2) {
-        constexpr float c = kShC2_0, dd = kShC2_2, e = kShC2_4;
-        out.basis[4] = c * x * y;
-        out.basis[5] = -c * y * z;
-        out.basis[6] = dd * (2.0f * z * z - x * x - y * y);
-        out.basis[7] = -c * x * z;
-        out.basis[8] = e * (x * x - y * y);
-        d[12] = c * y;
-        d[13] = c * x;
-        d[16] = -c * z;
-        d[17] = -c * y;
-        d[18] = -2.0f * dd * x;
-        d[19] = -2.0f * dd * y;
-        d[20] = 4.0f * dd * z;
-        d[21] = -c * z;
-        d[23] = -c * x;
-        d[24] = 2.0f * e * x;
-        d[25] = -2.0f * e * y;
-    }
-    if (Degree >= 3) {
-        constexpr float a = kShC3_0, b = kShC3_1, c = kShC3_2, dd = kShC3_3, e = kShC3_5;
-        out.basis[9] = a * y * (3.0f * x * x - y * y);
-        out.basis[10] = b * x * y * z;
-        out.basis[11] = c * y * (4.0f * z * z - x * x - y * y);
-        out.basis[12] = dd * z * (2.0f * z * z - 3.0f * x * x - 3.0f * y * y);
-        out.basis[13] = c * x * (4.0f * z * z - x * x - y * y);
-        out.basis[14] = e * z * (x * x - y * y);
-        out.basis[15] = a * x * (x * x - 3.0f * y * y);
-        d[27] = 6.0f * a * x * y;
-        d[28] = 3.0f * a * (x * x - y * y);
-        d[30] = b * y * z;
-        d[31] = b * x * z;
-        d[32] = b * x * y;
-        d[33] = -2.0f * c * x * y;
-        d[34] = c * (4.0f * z * z - x * x - 3.0f * y * y);
-        d[35] = 8.0f * c * y * z;
-        d[36] = -6.0f * dd * x * z;
-        d[37] = -6.0f * dd * y * z;
-        d[38] = dd * (6.0f * z * z - 3.0f * x * x - 3.0f * y * y);
-        d[39] = c * (4.0f * z * z - 3.0f * x * x - y * y);
-        d[40] = -2.0f * c * x * y;
-        d[41] = 8.0f * c * x * z;
-        d[42] = 2.0f * e * x * z;
-        d[43] = -2.0f * e * y * z;
-        d[44] = e * (x * x - y * y);
-        d[45] = 3.0f * a * (x * x - y * y);
-        d[46] = -6.0f * a * x * y;
-    }
+    out.x = x;
+    out.y = y;
+    out.z = z;
     float color_derivatives[9];
 #pragma unroll
     for (int channel = 0; channel < 3; ++channel) {
-        float value = 0.0f;
-        float derivative[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int j = 0; j < s; ++j) {
-            float coefficient = params[(kShIndex + channel * 16 + j) * n + id];
-            value += coefficient * out.basis[j];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                derivative[a] += coefficient * d[j * 3 + a];
-            }
-        }
-        out.color_ok[channel] = value + 0.5f >= 0.0f;
-        float radial = derivative[0] * x + derivative[1] * y + derivative[2] * z;
-        color_derivatives[channel * 3 + 0] = (derivative[0] - radial * x) * inverse_length;
-        color_derivatives[channel * 3 + 1] = (derivative[1] - radial * y) * inverse_length;
-        color_derivatives[channel * 3 + 2] = (derivative[2] - radial * z) * inverse_length;
+        out.color_ok[channel] = value[channel] + 0.5f >= 0.0f;
+        float radial = derivative[channel][0] * x + derivative[channel][1] * y + derivative[channel][2] * z;
+        color_derivatives[channel * 3 + 0] = (derivative[channel][0] - radial * x) * inverse_length;
+        color_derivatives[channel * 3 + 1] = (derivative[channel][1] - radial * y) * inverse_length;
+        color_derivatives[channel * 3 + 2] = (derivative[channel][2] - radial * z) * inverse_length;
     }
 
     float right_gradient[3];
@@ -396,7 +389,8 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
             else {
                 int channel = (row - kShIndex) / 16;
                 int k = (row - kShIndex) % 16;
-                g = (k < coefficients && mapped.color_ok[channel]) ? in[6 + channel] * mapped.basis[k] : 0.0f;
+                g = (k < coefficients && mapped.color_ok[channel])
+                    ? in[6 + channel] * sh_basis_value(k, mapped.x, mapped.y, mapped.z) : 0.0f;
             }
         }
         return g;
