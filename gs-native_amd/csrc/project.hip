@@ -12,10 +12,13 @@ namespace {
 
 constexpr float kNearDegenerate = 1e-3f;
 
+// Degree is a template parameter so that the SH loops unroll and all coefficient loads of a
+// Gaussian are in flight together (a runtime-bounded loop issued them one dependent step at a time).
+template<int Degree>
 __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ params, Camera camera, float2* means,
-    float4* conic_opacities, float* colors, float* depths, int4* tile_rects, int* record_counts, int count,
-    int sh_degree) {
+    float4* conic_opacities, float* colors, float* depths, int4* tile_rects, int* record_counts, int count) {
 #pragma clang fp contract(off)
+    constexpr int sh_degree = Degree;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         record_counts[count] = 0;
@@ -153,12 +156,21 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
             basis[15] = kShC3_6 * xs * (xx - 3.0f * yy);
         }
     }
-    int coeffs = (sh_degree + 1) * (sh_degree + 1);
+    constexpr int coeffs = (Degree + 1) * (Degree + 1);
+    float coefficients[3][coeffs];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+        for (int j = 0; j < coeffs; ++j) {
+            coefficients[ch][j] = params[(kShIndex + ch * kShCoeffCount + j) * n + i];
+        }
+    }
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
         float value = 0.0f;
+#pragma unroll
         for (int j = 0; j < coeffs; ++j) {
-            value += basis[j] * params[(kShIndex + ch * kShCoeffCount + j) * n + i];
+            value += basis[j] * coefficients[ch][j];
         }
         colors[static_cast<int64_t>(i) * 3 + ch] = fmaxf(value + 0.5f, 0.0f);
     }
@@ -184,8 +196,23 @@ extern "C" int gsn_project_gaussians(gsn_context* context, const float* params, 
     static_assert(sizeof(Camera) == sizeof(gsn_camera), "camera layout");
     memcpy(&cam, camera, sizeof(cam));
     StageScope scope(context, kStageProject);
-    project_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(params, cam, reinterpret_cast<float2*>(means),
-        reinterpret_cast<float4*>(conic_opacities), colors, depths, reinterpret_cast<int4*>(tile_rects), record_offsets,
-        count, clamp_sh_degree(sh_degree));
+    unsigned grid = grid_for(count, 256);
+    float2* m = reinterpret_cast<float2*>(means);
+    float4* q = reinterpret_cast<float4*>(conic_opacities);
+    int4* r = reinterpret_cast<int4*>(tile_rects);
+    switch (clamp_sh_degree(sh_degree)) {
+    case 0:
+        project_kernel<0><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r, record_offsets, count);
+        break;
+    case 1:
+        project_kernel<1><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r, record_offsets, count);
+        break;
+    case 2:
+        project_kernel<2><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r, record_offsets, count);
+        break;
+    default:
+        project_kernel<3><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r, record_offsets, count);
+        break;
+    }
     return check_launch("project_kernel");
 }
