@@ -283,13 +283,18 @@ constexpr int kGradientBlock = 256;
 
 template<int Degree, bool Fused>
 __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const float* __restrict__ partials,
-    const int* __restrict__ record_offsets, int count, int record_count, Camera cam, float* params, float* gradients, float* screen,
-    float* statistics, float* absolute_statistics, float* first, float* second, float position_learning_rate,
-    float first_correction, float second_correction) {
+    const int* __restrict__ record_offsets, const int4* __restrict__ rects, const float* __restrict__ depths,
+    const uint64_t* __restrict__ tile_stops, int count, int record_count, Camera cam, float* params, float* gradients,
+    float* screen, float* statistics, float* absolute_statistics, float* first, float* second,
+    float position_learning_rate, float first_correction, float second_correction) {
     // Each wave's 64 Gaussians own one contiguous run of partial records. The wave streams it
     // through its own LDS slice in 64-record chunks (coalesced 16-byte loads, the next chunk held
     // in registers while the current one is summed); each lane then sums its own records in
     // order. Waves never wait for each other (no block barrier).
+    //
+    // A Gaussian's records follow its tile rectangle row-major. The backward wrote no partial for
+    // a record its tile never reached (every pixel done first): such a record's slot holds stale
+    // data and is skipped, by comparing the Gaussian's (depth bits, id) key with the tile's stop.
     __shared__ float4 s_records[kGradientBlock / 64][64 * 3];
     const int lane = threadIdx.x & 63;
     float4* slice = s_records[threadIdx.x >> 6];
@@ -297,10 +302,19 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     const int wave_first = id - lane;
     const int64_t n = count;
     int begin = 0, end = 0;
+    uint64_t key = 0;
+    int tile = 0, column = 0, rect_width = 1;
     if (id < count) {
         begin = min(record_offsets[id], record_count);
         end = min(record_offsets[id + 1], record_count);
+        if (end > begin) {
+            const int4 rect = rects[id];
+            key = tile_order_key(depths, id);
+            tile = rect.y * cam.tile_count_x + rect.x;
+            rect_width = rect.z - rect.x;
+        }
     }
+    const int row_step = cam.tile_count_x - rect_width;
     const int wave_begin = wave_first < count ? min(record_offsets[wave_first], record_count) : 0;
     const int wave_end = wave_first < count ? min(record_offsets[min(wave_first + 64, count)], record_count) : 0;
     float in[11];
@@ -336,18 +350,25 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
         }
         const int last = min(end, next);
         for (int p = max(begin, chunk); p < last; ++p) {
-            const float4 a = slice[(p - chunk) * 3], b = slice[(p - chunk) * 3 + 1], c = slice[(p - chunk) * 3 + 2];
-            in[0] += a.x;
-            in[1] += a.y;
-            in[2] += a.z;
-            in[3] += a.w;
-            in[4] += b.x;
-            in[5] += b.y;
-            in[6] += b.z;
-            in[7] += b.w;
-            in[8] += c.x;
-            in[9] += c.y;
-            in[10] += c.z;
+            if (key < tile_stops[tile]) {
+                const float4 a = slice[(p - chunk) * 3], b = slice[(p - chunk) * 3 + 1], c = slice[(p - chunk) * 3 + 2];
+                in[0] += a.x;
+                in[1] += a.y;
+                in[2] += a.z;
+                in[3] += a.w;
+                in[4] += b.x;
+                in[5] += b.y;
+                in[6] += b.z;
+                in[7] += b.w;
+                in[8] += c.x;
+                in[9] += c.y;
+                in[10] += c.z;
+            }
+            ++tile;
+            if (++column == rect_width) {
+                column = 0;
+                tile += row_step;
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -477,26 +498,32 @@ __global__ __launch_bounds__(256) void zero_rows_kernel(float* first, float* sec
 
 template<bool Fused>
 int launch_gradients(gsn_context* context, const Camera& cam, int count, int record_count, int sh_degree,
-    const float* partials, const int* record_offsets, float* params, float* gradients, float* screen, float* statistics,
+    const float* partials, const int* record_offsets, const int* rects, const float* depths,
+    const uint64_t* tile_stops, float* params, float* gradients, float* screen, float* statistics,
     float* absolute_statistics, float* first, float* second, float lr, float c1, float c2) {
     unsigned grid = grid_for(count, kGradientBlock);
     hipStream_t stream = context->stream;
     StageScope scope(context, kStageGaussianGradients);
+    const int4* rect4 = reinterpret_cast<const int4*>(rects);
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
-        gaussian_gradients_kernel<0, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
+        gaussian_gradients_kernel<0, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, rect4, depths,
+            tile_stops, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     case 1:
-        gaussian_gradients_kernel<1, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
+        gaussian_gradients_kernel<1, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, rect4, depths,
+            tile_stops, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     case 2:
-        gaussian_gradients_kernel<2, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
+        gaussian_gradients_kernel<2, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, rect4, depths,
+            tile_stops, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     default:
-        gaussian_gradients_kernel<3, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, count, record_count, cam, params,
+        gaussian_gradients_kernel<3, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, rect4, depths,
+            tile_stops, count, record_count, cam, params,
             gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
         break;
     }
@@ -505,16 +532,16 @@ int launch_gradients(gsn_context* context, const Camera& cam, int count, int rec
 
 int validate_gradient_inputs(const gsn_context* context, const gsn_camera* camera, int count, const void* params,
     const int* ids, const int64_t* tile_offsets, int record_count, const int* record_offsets, const int* rects,
-    const float* means, const float* conics, const float* colors, const float* image, const float* image_gradients,
-    const float* statistics, const float* absolute_statistics) {
+    const float* means, const float* conics, const float* colors, const float* depths, const float* image,
+    const float* image_gradients, const float* statistics, const float* absolute_statistics) {
     if (!context || !camera || count < 0 || record_count < 0) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gaussian gradients: invalid arguments");
     }
     if (camera->width <= 0 || camera->height <= 0 || camera->tile_count_x <= 0 || camera->tile_count_y <= 0) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gaussian gradients: invalid camera");
     }
-    if (count > 0 && (!params || !tile_offsets || !record_offsets || !rects || !means || !conics || !colors || !image ||
-                         !image_gradients || !statistics || !absolute_statistics)) {
+    if (count > 0 && (!params || !tile_offsets || !record_offsets || !rects || !means || !conics || !colors ||
+                         !depths || !image || !image_gradients || !statistics || !absolute_statistics)) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gaussian gradients: null buffer");
     }
     if (record_count > 0 && !ids) {
@@ -529,11 +556,11 @@ int validate_gradient_inputs(const gsn_context* context, const gsn_camera* camer
 extern "C" int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* camera, int count, int sh_degree,
     const float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
     const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
-    const float* colors, const float* image, const float* image_gradients, float* gradients, float* screen_gradients,
+    const float* colors, const float* depths, const float* image, const float* image_gradients, float* gradients, float* screen_gradients,
     float* statistics, float* absolute_statistics) {
     using namespace gsn;
     GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
-        record_offsets, tile_rects, means, conic_opacities, colors, image, image_gradients, statistics,
+        record_offsets, tile_rects, means, conic_opacities, colors, depths, image, image_gradients, statistics,
         absolute_statistics));
     if (count == 0) {
         return GSN_OK;
@@ -545,23 +572,25 @@ extern "C" int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* ca
     if (!partials) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
+    const uint64_t* tile_stops = nullptr;
     if (record_count > 0) {
         GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
-            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count));
+            colors, depths, tile_rects, record_offsets, image, image_gradients, partials, record_count, &tile_stops));
     }
-    return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets, const_cast<float*>(params),
+    return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets, tile_rects, depths,
+        tile_stops, const_cast<float*>(params),
         gradients, screen_gradients, statistics, absolute_statistics, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
 }
 
 extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, int count, int sh_degree,
     float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
     const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
-    const float* colors, const float* image, const float* image_gradients, float* first_moments,
+    const float* colors, const float* depths, const float* image, const float* image_gradients, float* first_moments,
     float* second_moments, float position_learning_rate, float first_correction, float second_correction,
     float* statistics, float* absolute_statistics) {
     using namespace gsn;
     GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
-        record_offsets, tile_rects, means, conic_opacities, colors, image, image_gradients, statistics,
+        record_offsets, tile_rects, means, conic_opacities, colors, depths, image, image_gradients, statistics,
         absolute_statistics));
     if (count == 0) {
         return GSN_OK;
@@ -576,11 +605,13 @@ extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camer
     if (!partials) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
+    const uint64_t* tile_stops = nullptr;
     if (record_count > 0) {
         GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
-            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count));
+            colors, depths, tile_rects, record_offsets, image, image_gradients, partials, record_count, &tile_stops));
     }
-    return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, params, nullptr, nullptr,
+    return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, tile_rects, depths,
+        tile_stops, params, nullptr, nullptr,
         statistics, absolute_statistics, first_moments, second_moments, position_learning_rate, first_correction,
         second_correction);
 }
@@ -588,13 +619,14 @@ extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camer
 extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_camera* camera, int count,
     int sh_degree, const float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
     const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
-    const float* colors, const float* final_transmittance, const int32_t* last_contributor, const float* background,
+    const float* colors, const float* depths, const float* final_transmittance, const int32_t* last_contributor, const float* background,
     const float* image_gradients, float* gradients, float* screen_gradients, float* statistics,
     float* absolute_statistics) {
     using namespace gsn;
     // The image itself is not read by this form; validation only needs a non-null stand-in.
     GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
-        record_offsets, tile_rects, means, conic_opacities, colors, image_gradients, image_gradients, statistics,
+        record_offsets, tile_rects, means, conic_opacities, colors, depths, image_gradients, image_gradients,
+        statistics,
         absolute_statistics));
     if (count == 0) {
         return GSN_OK;
@@ -609,25 +641,28 @@ extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_cam
     if (!partials) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
+    const uint64_t* tile_stops = nullptr;
     if (record_count > 0) {
         StoredState stored{final_transmittance, last_contributor, {background[0], background[1], background[2]}};
         GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
-            colors, tile_rects, record_offsets, nullptr, image_gradients, partials, record_count, &stored));
+            colors, depths, tile_rects, record_offsets, nullptr, image_gradients, partials, record_count, &tile_stops,
+            &stored));
     }
-    return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets,
-        const_cast<float*>(params), gradients, screen_gradients, statistics, absolute_statistics, nullptr, nullptr,
+    return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets, tile_rects,
+        depths, tile_stops, const_cast<float*>(params), gradients, screen_gradients, statistics, absolute_statistics, nullptr, nullptr,
         0.0f, 0.0f, 0.0f);
 }
 
 extern "C" int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gsn_camera* camera, int count,
     int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
     const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
-    const float* colors, const float* final_transmittance, const int32_t* last_contributor, const float* background,
+    const float* colors, const float* depths, const float* final_transmittance, const int32_t* last_contributor, const float* background,
     const float* image_gradients, float* first_moments, float* second_moments, float position_learning_rate,
     float first_correction, float second_correction, float* statistics, float* absolute_statistics) {
     using namespace gsn;
     GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
-        record_offsets, tile_rects, means, conic_opacities, colors, image_gradients, image_gradients, statistics,
+        record_offsets, tile_rects, means, conic_opacities, colors, depths, image_gradients, image_gradients,
+        statistics,
         absolute_statistics));
     if (count == 0) {
         return GSN_OK;
@@ -643,13 +678,15 @@ extern "C" int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gs
     if (!partials) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
+    const uint64_t* tile_stops = nullptr;
     if (record_count > 0) {
         StoredState stored{final_transmittance, last_contributor, {background[0], background[1], background[2]}};
         GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
-            colors, tile_rects, record_offsets, nullptr, image_gradients, partials, record_count, &stored));
+            colors, depths, tile_rects, record_offsets, nullptr, image_gradients, partials, record_count, &tile_stops,
+            &stored));
     }
-    return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, params,
-        nullptr, nullptr, statistics, absolute_statistics, first_moments, second_moments, position_learning_rate,
+    return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, tile_rects,
+        depths, tile_stops, params, nullptr, nullptr, statistics, absolute_statistics, first_moments, second_moments, position_learning_rate,
         first_correction, second_correction);
 }
 

@@ -33,6 +33,7 @@ enum ScratchSlot {
     kSlotKnnPoints,
     kSlotMisc,
     kSlotSplats,
+    kSlotTileStops,
     kSlotCount
 };
 
@@ -119,11 +120,18 @@ struct StoredState {
 };
 
 // Tile rasteriser backward writing per-record partials (raster_backward.hip): the recompute form
-// when stored is null, else the back-to-front stored-state form.
+// when stored is null, else the back-to-front stored-state form. Records a tile did not reach
+// (every pixel done) get no partial; instead *tile_stops receives, per tile, the sort key
+// (depth bits << 32 | Gaussian id) of its first unreached record (all ones when it reached all),
+// and the per-Gaussian pass skips a record whose Gaussian's key is not below its tile's stop.
 int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
-    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
-    const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
-    const StoredState* stored = nullptr);
+    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const float* depths,
+    const int* rects, const int* record_offsets, const float* image, const float* image_gradients, float* partials,
+    int record_count, const uint64_t** tile_stops, const StoredState* stored = nullptr);
+
+__device__ __forceinline__ uint64_t tile_order_key(const float* depths, int id) {
+    return (static_cast<uint64_t>(__float_as_uint(depths[id])) << 32) | static_cast<uint32_t>(id);
+}
 
 inline unsigned grid_for(int64_t n, int block) {
     return static_cast<unsigned>((n + block - 1) / block);

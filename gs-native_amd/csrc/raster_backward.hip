@@ -58,9 +58,9 @@ __device__ __forceinline__ void store_partial(float* destination, const float4& 
 }
 
 __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restrict__ ids,
-    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats, Camera cam,
-    const float* __restrict__ image, const float* __restrict__ image_gradients, float* __restrict__ partials,
-    int tile_count, int record_count) {
+    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats, const float* __restrict__ depths,
+    Camera cam, const float* __restrict__ image, const float* __restrict__ image_gradients,
+    float* __restrict__ partials, uint64_t* __restrict__ tile_stops, int tile_count, int record_count) {
     __shared__ float s_partial[64 * kPartialFloats];
     const int tile = xcd_remap(blockIdx.x, tile_count);
     const int lane = threadIdx.x;
@@ -233,20 +233,10 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
         }
         __syncthreads();
     }
-    // Records after every pixel of the tile terminated contribute nothing.
-    for (; base < end; base += 64) {
-        const int64_t record = base + lane;
-        if (record < end) {
-            int64_t slot = partial_slot(splats[ids[record]].c, tile_x, tile_y);
-            if (slot < 0 || slot >= record_count) {
-                continue;
-            }
-            float* destination = partials + slot * kPartialFloats;
-            float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            store_partial(destination, zero);
-            store_partial(destination + 4, zero);
-            store_partial(destination + 8, zero);
-        }
+    // Records after every pixel of the tile terminated contribute nothing: instead of zero
+    // partials for them, the tile's stop key tells the per-Gaussian pass to skip them.
+    if (lane == 0) {
+        tile_stops[tile] = base < end ? tile_order_key(depths, ids[base]) : ~0ull;
     }
 }
 
@@ -259,10 +249,10 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
 // the same quantity as the recompute form's sum_c g_c (prefix_c + T_k c_c - C_c) / (1 - alpha_k).
 // Partials go to the same per-record slots.
 __global__ __launch_bounds__(64) void raster_backward_stored_kernel(const int* __restrict__ ids,
-    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats, Camera cam,
-    const float* __restrict__ final_transmittance, const int* __restrict__ last_contributor,
+    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats, const float* __restrict__ depths,
+    Camera cam, const float* __restrict__ final_transmittance, const int* __restrict__ last_contributor,
     const float* __restrict__ image_gradients, float bg_r, float bg_g, float bg_b, float* __restrict__ partials,
-    int tile_count, int record_count) {
+    uint64_t* __restrict__ tile_stops, int tile_count, int record_count) {
     __shared__ float s_partial[64 * kPartialFloats];
     const int tile = xcd_remap(blockIdx.x, tile_count);
     const int lane = threadIdx.x;
@@ -408,20 +398,9 @@ __global__ __launch_bounds__(64) void raster_backward_stored_kernel(const int* _
         }
         __syncthreads();
     }
-    // Records behind every pixel's last contributor contribute nothing.
-    for (int64_t base = stop; base < end; base += 64) {
-        const int64_t record = base + lane;
-        if (record < end) {
-            int64_t slot = partial_slot(splats[ids[record]].c, tile_x, tile_y);
-            if (slot < 0 || slot >= record_count) {
-                continue;
-            }
-            float* destination = partials + slot * kPartialFloats;
-            float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            store_partial(destination, zero);
-            store_partial(destination + 4, zero);
-            store_partial(destination + 8, zero);
-        }
+    // Records behind every pixel's last contributor contribute nothing (skipped by stop key).
+    if (lane == 0) {
+        tile_stops[tile] = stop < end ? tile_order_key(depths, ids[stop]) : ~0ull;
     }
 }
 
@@ -443,18 +422,20 @@ __global__ __launch_bounds__(64) void wave_sum12_selftest_kernel(const float* in
 } // namespace
 
 int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
-    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
-    const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
-    const StoredState* stored) {
+    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const float* depths,
+    const int* rects, const int* record_offsets, const float* image, const float* image_gradients, float* partials,
+    int record_count, const uint64_t** tile_stops, const StoredState* stored) {
     if (cam.tile_count_x > 0xffff || cam.tile_count_y > 0x7fff) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "raster backward: tile grid exceeds 65535 x 32767");
     }
     int64_t tiles = static_cast<int64_t>(cam.tile_count_x) * cam.tile_count_y;
     PackedSplat* splats = static_cast<PackedSplat*>(
         scratch(context, kSlotSplats, static_cast<size_t>(count > 0 ? count : 1) * sizeof(PackedSplat)));
-    if (!splats) {
+    uint64_t* stops = static_cast<uint64_t*>(scratch(context, kSlotTileStops, static_cast<size_t>(tiles) * 8));
+    if (!splats || !stops) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
+    *tile_stops = stops;
     StageScope scope(context, kStageRasterBackward);
     pack_splats_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(reinterpret_cast<const float2*>(means),
         reinterpret_cast<const float4*>(conics), colors, reinterpret_cast<const int4*>(rects), record_offsets, count,
@@ -462,13 +443,13 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
     GSN_TRY(check_launch("pack_splats_kernel"));
     if (stored) {
         raster_backward_stored_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets,
-            splats, cam, stored->final_transmittance, stored->last_contributor, image_gradients,
-            stored->background[0], stored->background[1], stored->background[2], partials, static_cast<int>(tiles),
-            record_count);
+            splats, depths, cam, stored->final_transmittance, stored->last_contributor, image_gradients,
+            stored->background[0], stored->background[1], stored->background[2], partials, stops,
+            static_cast<int>(tiles), record_count);
         return check_launch("raster_backward_stored_kernel");
     }
-    raster_backward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets, splats, cam,
-        image, image_gradients, partials, static_cast<int>(tiles), record_count);
+    raster_backward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets, splats, depths,
+        cam, image, image_gradients, partials, stops, static_cast<int>(tiles), record_count);
     return check_launch("raster_backward_kernel");
 }
 
