@@ -34,6 +34,7 @@ enum ScratchSlot {
     kSlotMisc,
     kSlotSplats,
     kSlotTileStops,
+    kSlotTileOrder,
     kSlotCount
 };
 
