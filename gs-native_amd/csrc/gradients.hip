@@ -287,56 +287,99 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     const uint64_t* __restrict__ tile_stops, int count, int record_count, Camera cam, float* params, float* gradients,
     float* screen, float* statistics, float* absolute_statistics, float* first, float* second,
     float position_learning_rate, float first_correction, float second_correction) {
-    // Each Gaussian owns the contiguous run of partial records of its tile rectangle (row-major).
-    // The backward wrote a partial only for the records its tile reached before every pixel was
-    // done; for the others it left the slot stale and recorded the tile's stop key instead. Each
-    // thread walks its rectangle, compares the Gaussian's (depth bits, id) key with the tile's
-    // stop, and loads only the partials that were written (at 1080p with a converged scene about
-    // 4% of the records). The records are summed in rectangle order, so results are identical
-    // run to run.
+    // Each Gaussian owns the contiguous run of partial records of its tile rectangle (row-major),
+    // so a wave's 64 Gaussians own one contiguous run. The backward wrote a partial only for the
+    // records its tile reached before every pixel was done; for the others it left the slot stale
+    // and recorded the tile's stop key instead (at 1080p with a converged scene about 4% of the
+    // records are reached). The wave walks its run 64 records at a time, one record per lane:
+    // the lane finds the record's Gaussian (binary search over the wave's run starts in LDS) and
+    // tile, compares the Gaussian's (depth bits, id) key with the tile's stop, and loads the
+    // partial only when it was written (coalesced, as neighbouring lanes hold neighbouring
+    // records). Each lane then adds the reached records of its own Gaussian from LDS, in record
+    // order, so results are identical run to run. Lanes stay busy whatever the spread of
+    // rectangle sizes within the wave, and waves never wait for each other.
+    __shared__ float4 s_records[kGradientBlock / 64][64 * 3];
+    __shared__ int s_begin[kGradientBlock / 64][64];
+    __shared__ int4 s_rect[kGradientBlock / 64][64];
+    __shared__ uint64_t s_key[kGradientBlock / 64][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int id = blockIdx.x * kGradientBlock + threadIdx.x;
     const int64_t n = count;
-    int begin = 0, end = 0;
+    int begin = record_count, end = record_count;
+    int4 rect = make_int4(0, 0, 1, 1);
+    uint64_t key = 0;
     if (id < count) {
         begin = min(record_offsets[id], record_count);
         end = min(record_offsets[id + 1], record_count);
+        if (end > begin) {
+            rect = rects[id];
+            key = tile_order_key(depths, id);
+        }
     }
+    s_begin[wave][lane] = begin;
+    s_rect[wave][lane] = make_int4(rect.x, rect.y, rect.z - rect.x, end);
+    s_key[wave][lane] = key;
+    const int last_lane = min(63, max(0, count - 1 - (id - lane)));
+    const int wave_begin = __shfl(begin, 0);
+    const int wave_end = __shfl(end, last_lane);
     float in[11];
 #pragma unroll
     for (int k = 0; k < 11; ++k) {
         in[k] = 0.0f;
     }
-    if (end > begin) {
-        const int4 rect = rects[id];
-        const uint64_t key = tile_order_key(depths, id);
-        const int width = rect.z - rect.x;
-        const int row_step = cam.tile_count_x - width;
-        const float4* records = reinterpret_cast<const float4*>(partials);
-        int tile = rect.y * cam.tile_count_x + rect.x;
-        int column = 0;
-        for (int p = begin; p < end; ++p) {
-            if (key < tile_stops[tile]) {
-                const float4 a = records[3 * static_cast<int64_t>(p)];
-                const float4 b = records[3 * static_cast<int64_t>(p) + 1];
-                const float4 c = records[3 * static_cast<int64_t>(p) + 2];
-                in[0] += a.x;
-                in[1] += a.y;
-                in[2] += a.z;
-                in[3] += a.w;
-                in[4] += b.x;
-                in[5] += b.y;
-                in[6] += b.z;
-                in[7] += b.w;
-                in[8] += c.x;
-                in[9] += c.y;
-                in[10] += c.z;
+    __builtin_amdgcn_wave_barrier();
+    for (int chunk = wave_begin; chunk < wave_end; chunk += 64) {
+        const int record = chunk + lane;
+        bool reached = false;
+        if (record < wave_end) {
+            // Owner: the last Gaussian of the wave whose run starts at or before the record
+            // (Gaussians with empty runs share the next one's start and precede it).
+            int owner = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                owner = s_begin[wave][owner + step] <= record ? owner + step : owner;
             }
-            ++tile;
-            if (++column == width) {
-                column = 0;
-                tile += row_step;
+            const int4 owner_rect = s_rect[wave][owner];  // x0, y0, width, run end
+            if (record < owner_rect.w) {
+                const int local = record - s_begin[wave][owner];
+                const int row = local / owner_rect.z;
+                const int tile =
+                    (owner_rect.y + row) * cam.tile_count_x + owner_rect.x + (local - row * owner_rect.z);
+                reached = s_key[wave][owner] < tile_stops[tile];
             }
         }
+        float4* slot = s_records[wave] + 3 * lane;
+        if (reached) {
+            const float4* source = reinterpret_cast<const float4*>(partials) + 3 * static_cast<int64_t>(record);
+            slot[0] = source[0];
+            slot[1] = source[1];
+            slot[2] = source[2];
+        }
+        uint64_t mine = __ballot(reached);
+        __builtin_amdgcn_wave_barrier();
+        // Keep only this lane's records [begin, end) within the chunk.
+        const int low = min(max(begin - chunk, 0), 64);
+        const int high = min(max(end - chunk, 0), 64);
+        mine &= high >= 64 ? ~0ull : ((1ull << high) - 1);
+        mine &= low >= 64 ? 0ull : ~((1ull << low) - 1);
+        while (mine) {
+            const int k = __builtin_ctzll(mine);
+            mine &= mine - 1;
+            const float4 a = s_records[wave][3 * k], b = s_records[wave][3 * k + 1], c = s_records[wave][3 * k + 2];
+            in[0] += a.x;
+            in[1] += a.y;
+            in[2] += a.z;
+            in[3] += a.w;
+            in[4] += b.x;
+            in[5] += b.y;
+            in[6] += b.z;
+            in[7] += b.w;
+            in[8] += c.x;
+            in[9] += c.y;
+            in[10] += c.z;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
     if (id >= count) {
         return;
