@@ -59,6 +59,14 @@ def algorithmic_bytes(n: int, v: float, r: float, p: int, t: int, k: int = 16) -
     }
 
 
+def fused_floor_bytes(n: int, v: float) -> dict:
+    """Compulsory bytes of the fused gradient + Adam pass itself, which never writes or re-reads a
+    58-row gradient buffer: Adam's parameter and moment rows read and written (1392N), the
+    statistics (16N), the run offsets (4N) and each visible Gaussian's rectangle and depth (20V).
+    The reached records' 48-byte partials come on top (data-dependent, not counted)."""
+    return {"gaussian_gradients": 1412 * n + 20 * v}
+
+
 def profiled_value(key: str, stage: str):
     """A per-kernel figure recorded by a committed rocprofv3 summary of this workload under
     profiles/ ("<key> <stage> = <value>" lines), or None: traffic_bytes_per_launch (FETCH_SIZE +
@@ -250,6 +258,10 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    floor = fused_floor_bytes(n_mean, v_mean).get(dominant)
+    if floor is not None:
+        result["roofline"]["fused_floor_bytes_per_launch"] = floor
+        result["roofline"]["frac_of_fused_floor"] = floor / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     result["roofline"]["traffic"] = profiled_value("traffic_bytes_per_launch", dominant)
     result["roofline"]["valu_share"] = profiled_value("valu_share", dominant)
     if args.calibration_copy_mib > 0:
