@@ -104,8 +104,8 @@ _PROTOTYPES = {
     "gsn_render_tiles": [_P, _CAM, _P, _P, _P, _P, _P, ctypes.POINTER(_F), _P],
     "gsn_image_gradients": [_P, _I, _I, _P, _P, _P],
     "gsn_image_metrics": [_P, _I, _I, _P, _P, ctypes.POINTER(ctypes.c_double)],
-    "gsn_gaussian_gradients": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
-    "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+    "gsn_gaussian_gradients": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                     _F, _F, _F, _P, _P],
     "gsn_adam_update": [_P, _I, _P, _P, _P, _P, _F, _F, _F],
     "gsn_remap_history": [_P, _P, _I, _I, _P, _P, _P, _P],
@@ -125,9 +125,9 @@ _PROTOTYPES = {
     "gsn_density_features": [_P, _P, _I, _P, _P, _P, _P, _I, _F, _F, _F, _P],
     "gsn_density_outcomes": [_P, _P, _P, ctypes.POINTER(ctypes.c_int32), _I, _P, _P, _I, _F, _P],
     "gsn_render_tiles_stored": [_P, _CAM, _P, _P, _P, _P, _P, ctypes.POINTER(_F), _P, _P, _P],
-    "gsn_gaussian_gradients_stored": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
+    "gsn_gaussian_gradients_stored": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
                                       ctypes.POINTER(_F), _P, _P, _P, _P, _P],
-    "gsn_gaussian_gradients_adam_stored": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
+    "gsn_gaussian_gradients_adam_stored": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
                                            ctypes.POINTER(_F), _P, _P, _P, _F, _F, _F, _P, _P],
     "gsn_mlp_sizes": [ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "gsn_mlp_sync_weights": [_P, _P, _P],
@@ -396,7 +396,7 @@ def gaussian_gradients(context: Context, projection: Projection, params: DeviceB
     p = projection
     call("gsn_gaussian_gradients", context.handle, ctypes.byref(p.camera), p.count, sh_degree, _p(params),
          _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets), _p(p.tile_rects),
-         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(p.depths), _p(image), _p(image_grad), _p(gradients), _p(screen),
+         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(gradients), _p(screen),
          _p(statistics), _p(absolute_statistics))
 
 
@@ -407,7 +407,7 @@ def gaussian_gradients_adam(context: Context, projection: Projection, params: De
     p = projection
     call("gsn_gaussian_gradients_adam", context.handle, ctypes.byref(p.camera), p.count, sh_degree, _p(params),
          _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets), _p(p.tile_rects),
-         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(p.depths), _p(image), _p(image_grad), _p(first), _p(second),
+         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(first), _p(second),
          position_lr, first_correction, second_correction, _p(statistics), _p(absolute_statistics))
 
 
@@ -490,6 +490,5 @@ def gaussian_gradients_stored(context: Context, projection: Projection, params: 
     bg = (ctypes.c_float * 3)(*background)
     call("gsn_gaussian_gradients_stored", context.handle, ctypes.byref(p.camera), p.count, sh_degree, _p(params),
          _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets), _p(p.tile_rects),
-         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(p.depths), _p(final_transmittance),
-         _p(last_contributor), bg,
+         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(final_transmittance), _p(last_contributor), bg,
          _p(image_grad), _p(gradients), _p(screen), _p(statistics), _p(absolute_statistics))

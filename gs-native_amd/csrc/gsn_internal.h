@@ -33,7 +33,7 @@ enum ScratchSlot {
     kSlotKnnPoints,
     kSlotMisc,
     kSlotSplats,
-    kSlotTileStops,
+    kSlotRecordFlags,
     kSlotTileOrder,
     kSlotCount
 };
@@ -63,6 +63,10 @@ struct gsn_context {
     void* slots[kSlotCount] = {};
     size_t slot_bytes[kSlotCount] = {};
     void* pinned = nullptr;  // small pinned host buffer for scalar readbacks
+    // Epoch stamp of the backward's record flags (1..255) and the flag buffer it was last cleared
+    // in: a byte equals the current epoch only if this step's backward reached that record.
+    uint8_t record_epoch = 0;
+    void* record_flags_cleared = nullptr;
     // Stage timing: event pairs recorded around kernel launches while enabled.
     bool timing = false;
     hipEvent_t* events = nullptr;
@@ -121,18 +125,14 @@ struct StoredState {
 };
 
 // Tile rasteriser backward writing per-record partials (raster_backward.hip): the recompute form
-// when stored is null, else the back-to-front stored-state form. Records a tile did not reach
-// (every pixel done) get no partial; instead *tile_stops receives, per tile, the sort key
-// (depth bits << 32 | Gaussian id) of its first unreached record (all ones when it reached all),
-// and the per-Gaussian pass skips a record whose Gaussian's key is not below its tile's stop.
+// when stored is null, else the back-to-front stored-state form. Only the records a tile reaches
+// before all its pixels are done get a partial; for each of them the backward also stores the
+// step's epoch into record_flags[slot] (*flags, *epoch are returned), so the per-Gaussian pass
+// reads exactly the partials written in this step.
 int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
-    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const float* depths,
-    const int* rects, const int* record_offsets, const float* image, const float* image_gradients, float* partials,
-    int record_count, const uint64_t** tile_stops, const StoredState* stored = nullptr);
-
-__device__ __forceinline__ uint64_t tile_order_key(const float* depths, int id) {
-    return (static_cast<uint64_t>(__float_as_uint(depths[id])) << 32) | static_cast<uint32_t>(id);
-}
+    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
+    const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
+    const uint8_t** flags, uint8_t* epoch, const StoredState* stored = nullptr);
 
 inline unsigned grid_for(int64_t n, int block) {
     return static_cast<unsigned>((n + block - 1) / block);

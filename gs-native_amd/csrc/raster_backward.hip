@@ -100,10 +100,10 @@ __device__ __forceinline__ void store_partial(float* destination, const float4& 
 }
 
 __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restrict__ ids,
-    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats, const float* __restrict__ depths,
+    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats,
     const int* __restrict__ tile_order, Camera cam, const float* __restrict__ image,
-    const float* __restrict__ image_gradients, float* __restrict__ partials, uint64_t* __restrict__ tile_stops,
-    int record_count) {
+    const float* __restrict__ image_gradients, float* __restrict__ partials, uint8_t* __restrict__ record_flags,
+    uint8_t epoch, int record_count) {
     __shared__ float s_partial[64 * kPartialFloats];
     const int tile = tile_order[blockIdx.x];
     const int lane = threadIdx.x;
@@ -273,14 +273,12 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
             store_partial(destination, source[0]);
             store_partial(destination + 4, source[1]);
             store_partial(destination + 8, source[2]);
+            record_flags[slot] = epoch;
         }
         __syncthreads();
     }
-    // Records after every pixel of the tile terminated contribute nothing: instead of zero
-    // partials for them, the tile's stop key tells the per-Gaussian pass to skip them.
-    if (lane == 0) {
-        tile_stops[tile] = base < end ? tile_order_key(depths, ids[base]) : ~0ull;
-    }
+    // Records after every pixel of the tile terminated contribute nothing; they get neither a
+    // partial nor this step's flag.
 }
 
 // Stored-state variant (no counterpart in the reference; kept for the memory comparison with the
@@ -292,10 +290,10 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
 // the same quantity as the recompute form's sum_c g_c (prefix_c + T_k c_c - C_c) / (1 - alpha_k).
 // Partials go to the same per-record slots.
 __global__ __launch_bounds__(64) void raster_backward_stored_kernel(const int* __restrict__ ids,
-    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats, const float* __restrict__ depths,
+    const int64_t* __restrict__ tile_offsets, const PackedSplat* __restrict__ splats,
     const int* __restrict__ tile_order, Camera cam, const float* __restrict__ final_transmittance,
     const int* __restrict__ last_contributor, const float* __restrict__ image_gradients, float bg_r, float bg_g,
-    float bg_b, float* __restrict__ partials, uint64_t* __restrict__ tile_stops, int record_count) {
+    float bg_b, float* __restrict__ partials, uint8_t* __restrict__ record_flags, uint8_t epoch, int record_count) {
     __shared__ float s_partial[64 * kPartialFloats];
     const int tile = tile_order[blockIdx.x];
     const int lane = threadIdx.x;
@@ -438,13 +436,12 @@ __global__ __launch_bounds__(64) void raster_backward_stored_kernel(const int* _
             store_partial(destination, source[0]);
             store_partial(destination + 4, source[1]);
             store_partial(destination + 8, source[2]);
+            record_flags[slot] = epoch;
         }
         __syncthreads();
     }
-    // Records behind every pixel's last contributor contribute nothing (skipped by stop key).
-    if (lane == 0) {
-        tile_stops[tile] = stop < end ? tile_order_key(depths, ids[stop]) : ~0ull;
-    }
+    // Records behind every pixel's last contributor contribute nothing; they get neither a
+    // partial nor this step's flag.
 }
 
 __global__ __launch_bounds__(64) void wave_sum12_selftest_kernel(const float* input, float* output) {
@@ -462,24 +459,47 @@ __global__ __launch_bounds__(64) void wave_sum12_selftest_kernel(const float* in
     }
 }
 
+__global__ __launch_bounds__(256) void clear_flags_kernel(uint32_t* flags, int64_t words) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < words) {
+        flags[i] = 0u;
+    }
+}
+
 } // namespace
 
 int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
-    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const float* depths,
-    const int* rects, const int* record_offsets, const float* image, const float* image_gradients, float* partials,
-    int record_count, const uint64_t** tile_stops, const StoredState* stored) {
+    const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
+    const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
+    const uint8_t** flags, uint8_t* epoch, const StoredState* stored) {
     if (cam.tile_count_x > 0xffff || cam.tile_count_y > 0x7fff) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "raster backward: tile grid exceeds 65535 x 32767");
     }
     int64_t tiles = static_cast<int64_t>(cam.tile_count_x) * cam.tile_count_y;
     PackedSplat* splats = static_cast<PackedSplat*>(
         scratch(context, kSlotSplats, static_cast<size_t>(count > 0 ? count : 1) * sizeof(PackedSplat)));
-    uint64_t* stops = static_cast<uint64_t*>(scratch(context, kSlotTileStops, static_cast<size_t>(tiles) * 8));
     int* order = static_cast<int*>(scratch(context, kSlotTileOrder, static_cast<size_t>(tiles) * 4));
-    if (!splats || !stops || !order) {
+    // One flag byte per record, read 16 at a time by the per-Gaussian pass (scratch slots are
+    // allocated in multiples of 256 bytes, so an aligned 16-byte read below the record count
+    // stays inside the allocation).
+    uint8_t* record_flags = static_cast<uint8_t*>(scratch(context, kSlotRecordFlags, static_cast<size_t>(record_count)));
+    if (!splats || !order || !record_flags) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
-    *tile_stops = stops;
+    // The epoch runs 1..255. After a wrap, or in a newly allocated buffer, every byte is cleared
+    // first, so that a stale byte never equals the current epoch.
+    uint8_t next = static_cast<uint8_t>(context->record_epoch + 1);
+    if (next == 0 || context->record_flags_cleared != record_flags) {
+        int64_t words = static_cast<int64_t>(context->slot_bytes[kSlotRecordFlags] / 4);
+        clear_flags_kernel<<<grid_for(words, 256), 256, 0, context->stream>>>(reinterpret_cast<uint32_t*>(record_flags),
+            words);
+        GSN_TRY(check_launch("clear_flags_kernel"));
+        context->record_flags_cleared = record_flags;
+        next = 1;
+    }
+    context->record_epoch = next;
+    *flags = record_flags;
+    *epoch = next;
     StageScope scope(context, kStageRasterBackward);
     pack_splats_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(reinterpret_cast<const float2*>(means),
         reinterpret_cast<const float4*>(conics), colors, reinterpret_cast<const int4*>(rects), record_offsets, count,
@@ -489,12 +509,13 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
     GSN_TRY(check_launch("tile_schedule_kernel"));
     if (stored) {
         raster_backward_stored_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets,
-            splats, depths, order, cam, stored->final_transmittance, stored->last_contributor, image_gradients,
-            stored->background[0], stored->background[1], stored->background[2], partials, stops, record_count);
+            splats, order, cam, stored->final_transmittance, stored->last_contributor, image_gradients,
+            stored->background[0], stored->background[1], stored->background[2], partials, record_flags, next,
+            record_count);
         return check_launch("raster_backward_stored_kernel");
     }
-    raster_backward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets, splats, depths,
-        order, cam, image, image_gradients, partials, stops, record_count);
+    raster_backward_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets, splats, order,
+        cam, image, image_gradients, partials, record_flags, next, record_count);
     return check_launch("raster_backward_kernel");
 }
 

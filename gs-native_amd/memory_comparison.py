@@ -77,8 +77,7 @@ def measure(form: str, gaussians: int, width: int, height: int, steps: int) -> d
             gsn.call("gsn_gaussian_gradients_adam_stored", context.handle, ctypes.byref(p.camera), p.count, 3,
                      device_params.ptr, p.tile_gaussian_ids.ptr, p.tile_offsets.ptr, p.record_count,
                      p.record_offsets.ptr, p.tile_rects.ptr, p.means.ptr, p.conic_opacities.ptr, p.colors.ptr,
-                     p.depths.ptr, state[0].ptr, state[1].ptr, background, image_grad.ptr, first.ptr, second.ptr,
-                     1e-4, c1, c2,
+                     state[0].ptr, state[1].ptr, background, image_grad.ptr, first.ptr, second.ptr, 1e-4, c1, c2,
                      stats.ptr, abs_stats.ptr)
         context.synchronize()
         if step:

@@ -60,8 +60,7 @@ bool Solver::Iterate(
                         reinterpret_cast<const int*>(renderer.GetTileRects()),
                         reinterpret_cast<const float*>(renderer.GetMeans()),
                         reinterpret_cast<const float*>(renderer.GetConicOpacities()),
-                        reinterpret_cast<const float*>(renderer.GetColors()), renderer.GetDepths(), image,
-                        image_gradients_, first_moments_,
+                        reinterpret_cast<const float*>(renderer.GetColors()), image, image_gradients_, first_moments_,
                         second_moments_, position_learning_rate, first_correction, second_correction,
                         projection_statistics_, absolute_projection_statistics_),
         "gsn_gaussian_gradients_adam");

@@ -150,9 +150,6 @@ int gsn_image_metrics(gsn_context* context, int width, int height, const float* 
  * statistics and the screen-to-parameter chain rule.
  * Replaces ComputeSplatGradientsKernel, CollectProjectionStatisticsKernel and
  * MapGaussianGradientsKernel<D> (solver.cu:837-899, 325-623).
- * depths[N] are the projection's camera-space depths (the tile order's keys): the backward
- * records, per tile, the first record it did not reach (all pixels done), so the per-Gaussian
- * pass skips those records instead of the backward writing zeros for them.
  * gradients[58][N]: parameter gradients (rows of unobserved Gaussians are zero).
  * screen_gradients[11][N] (optional): dmean2D(2), dconic(3), dlogit, dcolor(3), |dmean2D|(2).
  * statistics[2][N]: (|dmean2D| scaled by W/2,H/2; observed flag).
@@ -161,8 +158,8 @@ int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* camera, int c
     int sh_degree, const float* params, const int* tile_gaussian_ids,
     const int64_t* tile_offsets, int record_count, const int* record_offsets,
     const int* tile_rects, const float* means, const float* conic_opacities,
-    const float* colors, const float* depths, const float* image, const float* image_gradients,
-    float* gradients, float* screen_gradients, float* statistics, float* absolute_statistics);
+    const float* colors, const float* image, const float* image_gradients, float* gradients,
+    float* screen_gradients, float* statistics, float* absolute_statistics);
 
 /* Same as gsn_gaussian_gradients followed by gsn_adam_update, fused so that parameter
  * gradients never go to memory. Replaces Solver::Iterate's gradient + update stages
@@ -170,7 +167,7 @@ int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* camera, int c
 int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, int count,
     int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
     int record_count, const int* record_offsets, const int* tile_rects, const float* means,
-    const float* conic_opacities, const float* colors, const float* depths, const float* image,
+    const float* conic_opacities, const float* colors, const float* image,
     const float* image_gradients, float* first_moments, float* second_moments,
     float position_learning_rate, float first_correction, float second_correction,
     float* statistics, float* absolute_statistics);
@@ -196,13 +193,13 @@ int gsn_render_tiles_stored(gsn_context* context, const gsn_camera* camera,
 int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_camera* camera, int count,
     int sh_degree, const float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
     int record_count, const int* record_offsets, const int* tile_rects, const float* means,
-    const float* conic_opacities, const float* colors, const float* depths, const float* final_transmittance,
+    const float* conic_opacities, const float* colors, const float* final_transmittance,
     const int32_t* last_contributor, const float* background, const float* image_gradients,
     float* gradients, float* screen_gradients, float* statistics, float* absolute_statistics);
 int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gsn_camera* camera, int count,
     int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
     int record_count, const int* record_offsets, const int* tile_rects, const float* means,
-    const float* conic_opacities, const float* colors, const float* depths, const float* final_transmittance,
+    const float* conic_opacities, const float* colors, const float* final_transmittance,
     const int32_t* last_contributor, const float* background, const float* image_gradients,
     float* first_moments, float* second_moments, float position_learning_rate,
     float first_correction, float second_correction, float* statistics,
