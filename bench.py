@@ -59,12 +59,12 @@ def algorithmic_bytes(n: int, v: float, r: float, p: int, t: int, k: int = 16) -
     }
 
 
-def fused_floor_bytes(n: int, v: float) -> dict:
+def fused_floor_bytes(n: int, r: float) -> dict:
     """Compulsory bytes of the fused gradient + Adam pass itself, which never writes or re-reads a
     58-row gradient buffer: Adam's parameter and moment rows read and written (1392N), the
-    statistics (16N), the run offsets (4N) and each visible Gaussian's rectangle and depth (20V).
-    The reached records' 48-byte partials come on top (data-dependent, not counted)."""
-    return {"gaussian_gradients": 1412 * n + 20 * v}
+    statistics (16N), the run offsets (4N) and one flag byte per record (R). The reached records'
+    48-byte partials come on top (data-dependent, not counted)."""
+    return {"gaussian_gradients": 1412 * n + r}
 
 
 def profiled_value(key: str, stage: str):
@@ -258,7 +258,7 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
-    floor = fused_floor_bytes(n_mean, v_mean).get(dominant)
+    floor = fused_floor_bytes(n_mean, r_mean).get(dominant)
     if floor is not None:
         result["roofline"]["fused_floor_bytes_per_launch"] = floor
         result["roofline"]["frac_of_fused_floor"] = floor / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS

@@ -12,6 +12,12 @@ namespace {
 
 constexpr float kNearDegenerate = 1e-3f;
 
+// Parameter rows are read once per projection: non-temporal loads keep them from displacing data
+// that later kernels of the step read (measured 0.22 -> 0.17 ms at 3M Gaussians, SH degree 3).
+__device__ __forceinline__ float stream_load(const float* address) {
+    return __builtin_nontemporal_load(address);
+}
+
 // Degree is a template parameter so that the SH loops unroll and all coefficient loads of a
 // Gaussian are in flight together (a runtime-bounded loop issued them one dependent step at a time).
 template<int Degree>
@@ -28,9 +34,9 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     }
     record_counts[i] = 0;
     const int64_t n = count;
-    float px0 = params[(kPositionIndex + 0) * n + i];
-    float py0 = params[(kPositionIndex + 1) * n + i];
-    float pz0 = params[(kPositionIndex + 2) * n + i];
+    float px0 = stream_load(params + (kPositionIndex + 0) * n + i);
+    float py0 = stream_load(params + (kPositionIndex + 1) * n + i);
+    float pz0 = stream_load(params + (kPositionIndex + 2) * n + i);
     const float* Rc = camera.R_c_w;
     const float* tc = camera.t_c_w;
     float px = Rc[0] * px0 + Rc[1] * py0 + Rc[2] * pz0 + tc[0];
@@ -41,10 +47,11 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
         return;
     }
     float Rg[9];
-    rotation_from_axis_angle(params[(kRotationIndex + 0) * n + i], params[(kRotationIndex + 1) * n + i],
-        params[(kRotationIndex + 2) * n + i], Rg);
-    float scale[3] = {det_expf(params[(kLogScaleIndex + 0) * n + i]), det_expf(params[(kLogScaleIndex + 1) * n + i]),
-        det_expf(params[(kLogScaleIndex + 2) * n + i])};
+    rotation_from_axis_angle(stream_load(params + (kRotationIndex + 0) * n + i),
+        stream_load(params + (kRotationIndex + 1) * n + i), stream_load(params + (kRotationIndex + 2) * n + i), Rg);
+    float scale[3] = {det_expf(stream_load(params + (kLogScaleIndex + 0) * n + i)),
+        det_expf(stream_load(params + (kLogScaleIndex + 1) * n + i)),
+        det_expf(stream_load(params + (kLogScaleIndex + 2) * n + i))};
     float M[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -119,7 +126,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
         conic.y = -b * det_inv;
         conic.z = a * det_inv;
     }
-    conic.w = 1.0f / (1.0f + det_expf(-params[kOpacityIndex * n + i]));
+    conic.w = 1.0f / (1.0f + det_expf(-stream_load(params + kOpacityIndex * n + i)));
     means[i] = make_float2(u, v);
     conic_opacities[i] = conic;
 
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     for (int ch = 0; ch < 3; ++ch) {
 #pragma unroll
         for (int j = 0; j < coeffs; ++j) {
-            coefficients[ch][j] = params[(kShIndex + ch * kShCoeffCount + j) * n + i];
+            coefficients[ch][j] = stream_load(params + (kShIndex + ch * kShCoeffCount + j) * n + i);
         }
     }
 #pragma unroll
