@@ -318,24 +318,51 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     }
     const float4* records = reinterpret_cast<const float4*>(partials);
     float4* slots = s_records[wave];
+    // Flags of the first pass; each pass then loads the next pass's flags while its partials are
+    // in flight, so a pass waits on memory about once. Flag reads past the run clamp to its last
+    // record (they are never used).
+    static_assert(kChunks == 2, "the staging below is written out for two chunks");
+    uint8_t flag0 = 0, flag1 = 0;
+    if (wave_begin < wave_end) {
+        flag0 = record_flags[min(wave_begin + lane, wave_end - 1)];
+        flag1 = record_flags[min(wave_begin + 64 + lane, wave_end - 1)];
+    }
     for (int pass = wave_begin; pass < wave_end; pass += kChunks * 64) {
         bool reached[kChunks];
-#pragma unroll
-        for (int c = 0; c < kChunks; ++c) {
-            const int record = pass + c * 64 + lane;
-            reached[c] = record < wave_end && record_flags[record] == epoch;
-        }
+        reached[0] = pass + lane < wave_end && flag0 == epoch;
+        reached[1] = pass + 64 + lane < wave_end && flag1 == epoch;
         uint64_t ballots[kChunks];
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) {
             ballots[c] = __ballot(reached[c]);
-            if (reached[c]) {
-                const int64_t source = 3 * static_cast<int64_t>(pass + c * 64 + lane);
-                float4* slot = slots + 3 * (c * 64 + lane);
-                slot[0] = records[source];
-                slot[1] = records[source + 1];
-                slot[2] = records[source + 2];
-            }
+        }
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 a0 = zero, b0 = zero, c0 = zero, a1 = zero, b1 = zero, c1 = zero;
+        const int64_t source = 3 * static_cast<int64_t>(pass + lane);
+        if (reached[0]) {
+            a0 = records[source];
+            b0 = records[source + 1];
+            c0 = records[source + 2];
+        }
+        if (reached[1]) {
+            a1 = records[source + 3 * 64];
+            b1 = records[source + 3 * 64 + 1];
+            c1 = records[source + 3 * 64 + 2];
+        }
+        const int next = pass + kChunks * 64;
+        if (next < wave_end) {
+            flag0 = record_flags[min(next + lane, wave_end - 1)];
+            flag1 = record_flags[min(next + 64 + lane, wave_end - 1)];
+        }
+        if (reached[0]) {
+            slots[3 * lane] = a0;
+            slots[3 * lane + 1] = b0;
+            slots[3 * lane + 2] = c0;
+        }
+        if (reached[1]) {
+            slots[3 * (64 + lane)] = a1;
+            slots[3 * (64 + lane) + 1] = b1;
+            slots[3 * (64 + lane) + 2] = c1;
         }
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
