@@ -51,12 +51,12 @@ def log(message: str) -> None:
 
 class Scene:
     def __init__(self, context, seed: int, truth_count: int, init_count: int, width: int, height: int,
-                 cameras: int, heldout: int):
+                 cameras: int, heldout: int, colour_noise: float = 0.05):
         self.seed = seed
-        truth = scenes.blob_scene(truth_count, seed=seed, sh_degree=3)
+        truth = scenes.blob_scene(truth_count, seed=seed, sh_degree=3, colour_noise=colour_noise)
         _, d2 = gsn.knn3(context, truth[0:3].T.copy())
         truth[3:6] = scenes.scales_from_mean_dist2(d2)[None, :]
-        points = scenes.blob_scene(init_count, seed=seed, sh_degree=0)
+        points = scenes.blob_scene(init_count, seed=seed, sh_degree=0, colour_noise=colour_noise)
         init = np.zeros((58, init_count), np.float32)
         init[0:3] = points[0:3]
         _, d2 = gsn.knn3(context, points[0:3].T.copy())
@@ -146,6 +146,7 @@ def train_network(context, features, labels, weights, epochs: int, batch: int, l
     y = gsn.DeviceBuffer.from_numpy(labels[validation:].astype(np.int32))
     w = gsn.DeviceBuffer.from_numpy(balanced[validation:])
     n = x.shape[0]
+    batch = min(batch, n)  # small collections: one batch per epoch
     steps = 0
     losses = []
     for epoch in range(epochs):
@@ -210,6 +211,8 @@ def main() -> int:
     parser.add_argument("--useful-opacity", type=float, default=0.02,
                         help="a densified Gaussian's outputs count as useful when observed in the next interval "
                              "and at least this opaque")
+    parser.add_argument("--colour-noise", type=float, default=0.05,
+                        help="per-Gaussian colour noise of the truth scenes (their fine texture)")
     parser.add_argument("--quick", action="store_true", help="small sizes for a functional check")
     parser.add_argument("--out", type=str, default="")
     args = parser.parse_args()
@@ -228,7 +231,8 @@ def main() -> int:
     outcome_log = trainer_module.OutcomeLog(keep_fraction=args.keep_fraction, seed=1)
     collect_runs = []
     for seed in range(args.train_seeds):
-        scene = Scene(context, 100 + seed, args.collect_truth, args.collect_init, cw, ch, args.collect_cameras, 1)
+        scene = Scene(context, 100 + seed, args.collect_truth, args.collect_init, cw, ch, args.collect_cameras, 1,
+                      args.colour_noise)
         trainer, curve = run(context, scene, args.collect_iterations, args.cap, outcome_log=outcome_log,
                              useful_opacity=args.useful_opacity)
         samples = sum(f.shape[0] for f in outcome_log.features)
@@ -248,13 +252,14 @@ def main() -> int:
     evaluations = []
     for k in range(args.eval_seeds):
         seed = 900 + k
-        scene = Scene(context, seed, args.eval_truth, args.eval_init, ew, eh, args.eval_cameras, args.eval_heldout)
+        scene = Scene(context, seed, args.eval_truth, args.eval_init, ew, eh, args.eval_cameras, args.eval_heldout,
+                      args.colour_noise)
         results = {}
         for name, policy in (("rule", None), ("learned", network)):
             trainer, curve = run(context, scene, args.eval_iterations, args.cap, policy=policy,
                                  eval_every=args.eval_every)
             results[name] = {"curve": curve, "final_count": trainer.count, "final_psnr": curve[-1][3],
-                             "seconds": curve[-1][1]}
+                             "seconds": curve[-1][1], "policy_fallbacks": trainer.policy_fallbacks}
             log(f"eval seed {seed} {name}: PSNR {curve[-1][3]:.3f} dB, {trainer.count} Gaussians, {curve[-1][1]:.1f} s")
             del trainer
         rule, learned = results["rule"], results["learned"]

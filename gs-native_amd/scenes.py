@@ -120,10 +120,11 @@ def sphere_cameras(count: int, radius: float, seed: int, hemisphere: bool = Fals
     return poses
 
 
-def blob_scene(count: int, seed: int, blobs: int = 64, sh_degree: int = 3) -> np.ndarray:
+def blob_scene(count: int, seed: int, blobs: int = 64, sh_degree: int = 3, colour_noise: float = 0.05) -> np.ndarray:
     """Gaussians sampled from a seeded mixture of anisotropic blobs inside the unit ball (the
     large-scale workload): positions from the mixture, log-scales set later from 3-NN distances,
-    opacity logit N(0, 1), SH DC from per-blob colours plus noise, higher bands N(0, 0.05)."""
+    opacity logit N(0, 1), SH DC from per-blob colours plus N(0, colour_noise) per Gaussian (the
+    scene's fine texture), higher bands N(0, 0.05)."""
     rng = np.random.default_rng(seed)
     centres = rng.uniform(-1.5, 1.5, size=(blobs, 3))
     axes = rng.uniform(0.05, 0.4, size=(blobs, 3))
@@ -138,7 +139,7 @@ def blob_scene(count: int, seed: int, blobs: int = 64, sh_degree: int = 3) -> np
     p[3:6] = -4.0
     p[6:9] = random_rotations(rng, count).T
     p[9] = rng.normal(0.0, 1.0, size=count)
-    rgb = np.clip(colours[labels] + rng.normal(0.0, 0.05, size=(count, 3)), 0.0, 1.0)
+    rgb = np.clip(colours[labels] + rng.normal(0.0, colour_noise, size=(count, 3)), 0.0, 1.0)
     for c in range(3):
         p[10 + 16 * c] = (rgb[:, c] - 0.5) / SH_C0
         if sh_degree > 0:

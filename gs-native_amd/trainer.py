@@ -67,6 +67,7 @@ class Trainer:
                  policy=None, outcome_log: OutcomeLog | None = None, useful_opacity: float = 0.02):
         self.context = context
         self.policy = policy
+        self.policy_fallbacks = 0  # density steps where the growth guard handed the decision back to the rule
         self.outcome_log = outcome_log
         self.useful_opacity = useful_opacity
         self.features_enabled = policy is not None or outcome_log is not None
@@ -184,8 +185,8 @@ class Trainer:
             logits = DeviceBuffer((count, 3), np.float32)
             self.policy.forward(features, count, logits)
 
-        def count_changes(threshold, absolute):
-            if logits is not None and np.isfinite(threshold):
+        def count_changes(threshold, absolute, use_policy=True):
+            if use_policy and logits is not None and np.isfinite(threshold):
                 call("gsn_density_count_proposed", self.context.handle, self.params.ptr, count, self.sums.ptr,
                      self.absolute_sums.ptr, self.observations.ptr, logits.ptr, split_threshold, min_opacity, world,
                      int(max_screen > 0), decisions.ptr, offsets.ptr, totals)
@@ -197,6 +198,16 @@ class Trainer:
             return t, t[0] + t[1] + 2 * t[3]
 
         t, out_count = count_changes(gradient_threshold, absolute_threshold)
+        if logits is not None:
+            # Growth guard: a proposal that adds more than twice what the plain rule would (and more
+            # than 1% of the scene) is not followed; the rule decides this step instead.
+            proposed_added = out_count - count
+            _, rule_count = count_changes(gradient_threshold, absolute_threshold, use_policy=False)
+            if proposed_added > max(2 * max(rule_count - count, 0), count // 100):
+                self.policy_fallbacks += 1
+                t, out_count = count_changes(gradient_threshold, absolute_threshold, use_policy=False)
+            else:
+                t, out_count = count_changes(gradient_threshold, absolute_threshold)
         if self.max_count is not None and out_count > self.max_count:
             # Capacity cap: no clones or splits this round, pruning only.
             t, out_count = count_changes(float("inf"), float("inf"))

@@ -160,3 +160,14 @@ def test_trainer_logs_outcomes_and_runs_a_policy(gsn, context, scenes):
     context.synchronize()
     assert keep_all.history and all(changes[0] == 0 for _, _, _, changes in keep_all.history)
     assert keep_all.count <= params.shape[1]
+    # a network that always answers split proposes far more growth than the rule: the growth guard
+    # hands every density step back to the rule, so the run repeats the rule run exactly
+    parts["b3"][:] = [0.0, 0.0, 1.0]
+    split_all = trainer_module.Trainer(context, params, cams, targets, 5.0,
+                                       policy=gsn.ProposalNetwork(context, mlp.pack(parts)))
+    for _ in range(800):
+        split_all.iteration()
+    context.synchronize()
+    assert split_all.policy_fallbacks == len(split_all.history) > 0
+    assert split_all.history == trainer.history
+    assert split_all.count == trainer.count
