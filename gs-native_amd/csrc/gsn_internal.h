@@ -63,10 +63,12 @@ struct gsn_context {
     void* slots[kSlotCount] = {};
     size_t slot_bytes[kSlotCount] = {};
     void* pinned = nullptr;  // small pinned host buffer for scalar readbacks
-    // Epoch stamp of the backward's record flags (1..255) and the flag buffer it was last cleared
-    // in: a byte equals the current epoch only if this step's backward reached that record.
+    // Epoch stamp of the backward's record flags (1..255) and the flag buffer (address and size:
+    // a regrown buffer may reuse the address) it was last cleared in: a byte equals the current
+    // epoch only if this step's backward reached that record.
     uint8_t record_epoch = 0;
     void* record_flags_cleared = nullptr;
+    size_t record_flags_cleared_bytes = 0;
     // Stage timing: event pairs recorded around kernel launches while enabled.
     bool timing = false;
     hipEvent_t* events = nullptr;

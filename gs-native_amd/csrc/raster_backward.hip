@@ -489,12 +489,15 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
     // The epoch runs 1..255. After a wrap, or in a newly allocated buffer, every byte is cleared
     // first, so that a stale byte never equals the current epoch.
     uint8_t next = static_cast<uint8_t>(context->record_epoch + 1);
-    if (next == 0 || context->record_flags_cleared != record_flags) {
-        int64_t words = static_cast<int64_t>(context->slot_bytes[kSlotRecordFlags] / 4);
+    const size_t flag_bytes = context->slot_bytes[kSlotRecordFlags];
+    if (next == 0 || context->record_flags_cleared != record_flags ||
+        context->record_flags_cleared_bytes != flag_bytes) {
+        int64_t words = static_cast<int64_t>(flag_bytes / 4);
         clear_flags_kernel<<<grid_for(words, 256), 256, 0, context->stream>>>(reinterpret_cast<uint32_t*>(record_flags),
             words);
         GSN_TRY(check_launch("clear_flags_kernel"));
         context->record_flags_cleared = record_flags;
+        context->record_flags_cleared_bytes = flag_bytes;
         next = 1;
     }
     context->record_epoch = next;
