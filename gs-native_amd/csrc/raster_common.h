@@ -32,7 +32,10 @@ __device__ __forceinline__ int readlane_i(int v, int lane) {
 // Margins: tau + 0.25 (28% on the alpha threshold); 2 tau is further raised by 1e-5 times the
 // magnitude of the three terms of q at the extents, which bounds the FP32 cancellation error of
 // the per-pixel q for elongated splats (~10 ulp of the largest term); extents get 2% + 0.5 px.
-// Non-finite inputs keep every band.
+// Per band, the region's x-range is taken over the band's rows only (the slab's extremes of the
+// ellipse's chords; the ellipse's own extreme point when it lies inside the slab), with the same
+// 2% + 0.5 px margin and never wider than the whole-tile test: a band is kept only if that range
+// meets the tile's 16 columns. Non-finite inputs keep every band.
 __device__ __forceinline__ int band_mask(float mx, float my, float4 conic, int x0, int y0) {
     float a = conic.x, b = conic.y, c = conic.z, o = conic.w;
     if (!(o >= kAlphaMin)) {
@@ -50,20 +53,52 @@ __device__ __forceinline__ int band_mask(float mx, float my, float4 conic, int x
     float ey2 = two_tau * syy;
     float term_bound = (fabsf(a) + fabsf(b)) * ex2 + (fabsf(b) + fabsf(c)) * ey2;
     two_tau += 2e-5f * term_bound;
-    float ext_x = sqrtf(two_tau * sxx) * 1.02f + 0.5f;
-    float ext_y = sqrtf(two_tau * syy) * 1.02f + 0.5f;
+    const float ex_raw = sqrtf(two_tau * sxx);
+    const float ey_raw = sqrtf(two_tau * syy);
+    float ext_x = ex_raw * 1.02f + 0.5f;
+    float ext_y = ey_raw * 1.02f + 0.5f;
     if (!isfinite(ext_x) || !isfinite(ext_y)) {
         return 0xf;
     }
-    if (mx + ext_x < static_cast<float>(x0) || mx - ext_x > static_cast<float>(x0 + 15)) {
+    const float left = static_cast<float>(x0);
+    const float right = static_cast<float>(x0 + 15);
+    if (mx + ext_x < left || mx - ext_x > right) {
         return 0;
     }
+    // Offsets d = (splat - pixel); the region's chord at row offset dy spans
+    // dx in [(-b dy - w) / a, (-b dy + w) / a], w = sqrt(2 tau a - det dy^2). Its largest dx is
+    // ex_raw at dy = -b ex_raw / c, its smallest -ex_raw at dy = b ex_raw / c; the chord ends
+    // are concave / convex in dy, so over a slab without that point the extreme is at an end.
+    const float inverse_a = 1.0f / a;
+    const float dy_of_max = -b * ex_raw / c;
+    const float two_tau_a = two_tau * a;
+    const float x_margin = 0.02f * ex_raw + 0.5f;
     int mask = 0;
 #pragma unroll
     for (int band = 0; band < 4; ++band) {
         float lo = static_cast<float>(y0 + 4 * band);
         float hi = lo + 3.0f;
-        if (my + ext_y >= lo && my - ext_y <= hi) {
+        if (!(my + ext_y >= lo && my - ext_y <= hi)) {
+            continue;
+        }
+        // Row offsets of the band's pixels, clamped to the region's raw y-extent.
+        const float s0 = fmaxf(my - hi - 0.05f, -ey_raw);
+        const float s1 = fminf(my - lo + 0.05f, ey_raw);
+        bool keep = true;
+        if (s0 <= s1) {
+            const float w0 = sqrtf(fmaxf(two_tau_a - det * s0 * s0, 0.0f));
+            const float w1 = sqrtf(fmaxf(two_tau_a - det * s1 * s1, 0.0f));
+            float dx_max = (dy_of_max >= s0 && dy_of_max <= s1) ? ex_raw : fmaxf(-b * s0 + w0, -b * s1 + w1) * inverse_a;
+            float dx_min = (-dy_of_max >= s0 && -dy_of_max <= s1) ? -ex_raw
+                                                                  : fminf(-b * s0 - w0, -b * s1 - w1) * inverse_a;
+            dx_max = fminf(dx_max + x_margin, ext_x);
+            dx_min = fmaxf(dx_min - x_margin, -ext_x);
+            // Pixel columns mx - dx for dx in [dx_min, dx_max] against the tile's columns.
+            if (isfinite(dx_max) && isfinite(dx_min)) {
+                keep = mx - dx_max <= right && mx - dx_min >= left;
+            }
+        }
+        if (keep) {
             mask |= 1 << band;
         }
     }
