@@ -524,6 +524,36 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
 
 } // namespace gsn
 
+namespace gsn {
+namespace {
+
+__global__ __launch_bounds__(256) void band_mask_selftest_kernel(const float* splats, const int* origins, int count,
+    int* masks) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) {
+        return;
+    }
+    const float* s = splats + static_cast<int64_t>(i) * 6;
+    masks[i] = band_mask(s[0], s[1], make_float4(s[2], s[3], s[4], s[5]), origins[2 * i], origins[2 * i + 1]);
+}
+
+} // namespace
+} // namespace gsn
+
+// Diagnostic: the rasterisers' band mask on `count` (splat, tile) cases.
+extern "C" int gsn_selftest_band_mask(gsn_context* context, const float* splats, const int* origins, int count,
+    int* masks) {
+    using namespace gsn;
+    if (!context || count < 0 || (count > 0 && (!splats || !origins || !masks))) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_selftest_band_mask: invalid arguments");
+    }
+    if (count == 0) {
+        return GSN_OK;
+    }
+    band_mask_selftest_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(splats, origins, count, masks);
+    return check_launch("band_mask_selftest_kernel");
+}
+
 // Diagnostic: runs the wave reduction on one wave. input float[64][12] -> output float[64][3].
 extern "C" int gsn_selftest_wave_sum12(gsn_context* context, const float* input, float* output) {
     using namespace gsn;

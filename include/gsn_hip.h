@@ -95,6 +95,13 @@ int gsn_context_stage_times(gsn_context* context, double* milliseconds, int* lau
  * fields 3*(l/16) .. 3*(l/16)+2. No counterpart in the reference. */
 int gsn_selftest_wave_sum12(gsn_context* context, const float* input, float* output);
 
+/* Diagnostic: the rasterisers' per-tile band mask. splats float[count][6] = (mean x, mean y,
+ * conic a, b, c, opacity) in the layout of the projection's means / conic_opacities; origins
+ * int[count][2] = tile pixel origin (x0, y0); masks int[count]: bit b set when rows
+ * y0+4b .. y0+4b+3 of the 16x16 tile may hold a pixel with alpha >= 1/255. No counterpart in the
+ * reference (its rasteriser tests every pixel of the tile). */
+int gsn_selftest_band_mask(gsn_context* context, const float* splats, const int* origins, int count, int* masks);
+
 /* Events on the context stream, for timing. */
 int gsn_event_create(void** event);
 int gsn_event_destroy(void* event);

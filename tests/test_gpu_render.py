@@ -29,6 +29,50 @@ def test_wave_sum12_lane_layout(gsn, context):
         np.testing.assert_allclose(result[lane], totals[3 * row: 3 * row + 3], rtol=1e-5, atol=1e-5)
 
 
+def test_band_mask_is_conservative(gsn, context):
+    """The device band mask never drops a band that holds a pixel with alpha >= 1/255 (checked in
+    FP64 with the per-pixel rule of the rasteriser, at a threshold 0.1% below 1/255), over random
+    ellipses from 0.3 to 400 px in any orientation and opacity, placed around random tiles; and it
+    does drop bands (it is not the trivial all-ones mask)."""
+    rng = np.random.default_rng(5)
+    count = 40000
+    s1, s2 = np.exp(rng.uniform(np.log(0.3), np.log(400.0), (2, count)))
+    theta = rng.uniform(0.0, np.pi, count)
+    cos, sin = np.cos(theta), np.sin(theta)
+    xx = cos * cos * s1 * s1 + sin * sin * s2 * s2 + 0.3
+    yy = sin * sin * s1 * s1 + cos * cos * s2 * s2 + 0.3
+    xy = cos * sin * (s1 * s1 - s2 * s2)
+    det = xx * yy - xy * xy
+    a, b, c = yy / det, -xy / det, xx / det
+    opacity = np.where(rng.random(count) < 0.5, rng.uniform(1 / 255, 0.05, count), rng.uniform(0.05, 1.0, count))
+    means = rng.uniform(0.0, 64.0, (count, 2))
+    origins = 16 * rng.integers(-3, 8, (count, 2)).astype(np.int32)
+    splats = np.stack([means[:, 0], means[:, 1], a, b, c, opacity], axis=1).astype(np.float32)
+    masks = gsn.DeviceBuffer((count,), np.int32)
+    # The device buffers stay referenced until the kernel has finished.
+    device_splats = gsn.DeviceBuffer.from_numpy(splats)
+    device_origins = gsn.DeviceBuffer.from_numpy(origins)
+    gsn.call("gsn_selftest_band_mask", context.handle, device_splats.ptr, device_origins.ptr, count, masks.ptr)
+    context.synchronize()
+    del device_splats, device_origins
+    masks = masks.numpy()
+    s = splats.astype(np.float64)
+    columns = np.arange(16.0)[None, :]
+    rows = np.arange(4.0)[:, None]
+    dropped = 0
+    for i in range(count):
+        for band in range(4):
+            if (masks[i] >> band) & 1:
+                continue
+            dropped += 1
+            dx = s[i, 0] - (origins[i, 0] + columns)
+            dy = s[i, 1] - (origins[i, 1] + 4 * band + rows)
+            power = -0.5 * (s[i, 2] * dx * dx + s[i, 4] * dy * dy) - s[i, 3] * dx * dy
+            alpha = np.minimum(0.99, s[i, 5] * np.exp(power))
+            assert not np.any((power <= 0.0) & (alpha >= (1.0 / 255.0) * (1.0 - 1e-3))), (i, band)
+    assert dropped > count  # most cases lie away from the tile
+
+
 def _compare_projection(run, expected):
     actual = run.projected()
     counts = expected["record_counts"][:-1]
