@@ -120,7 +120,8 @@ def run(context, scene: Scene, iterations: int, cap: int, policy=None, outcome_l
     return trainer, curve
 
 
-def train_network(context, features, labels, weights, epochs: int, batch: int, learning_rate: float, seed: int):
+def train_network(context, features, labels, weights, epochs: int, batch: int, learning_rate: float, seed: int,
+                  calibrate: bool = True):
     """Standardises the features, trains on the GPU, folds the standardisation into W1/b1 and
     returns (params, report)."""
     rng = np.random.default_rng(seed)
@@ -169,14 +170,41 @@ def train_network(context, features, labels, weights, epochs: int, batch: int, l
     vx = gsn.DeviceBuffer.from_numpy(np.ascontiguousarray(features[:validation], np.float32))
     logits = gsn.DeviceBuffer((validation, 3), np.float32)
     deployed.forward(vx, validation, logits)
-    predicted = np.argmax(logits.numpy(), axis=1)
+    validation_logits = logits.numpy().astype(np.float64)
     truth = labels[:validation]
+    biases = np.zeros(3)
+    if calibrate:
+        biases = calibrate_biases(validation_logits, truth, weights[:validation])
+        folded[-3:] += biases.astype(np.float32)  # b3 is the last layer's bias
+    predicted = np.argmax(validation_logits + biases[None, :], axis=1)
     confusion = [[int(np.sum((truth == a) & (predicted == b))) for b in range(3)] for a in range(3)]
     report = {"samples": int(features.shape[0]), "validation": int(validation), "steps": steps,
               "losses": [float(v) for v in losses], "validation_accuracy": float(np.mean(predicted == truth)),
-              "validation_confusion": confusion,
+              "validation_confusion": confusion, "output_biases": [float(v) for v in biases],
               "label_counts": [int(np.sum(labels == c)) for c in range(3)]}
     return folded.astype(np.float32), report
+
+
+def calibrate_biases(logits: np.ndarray, labels: np.ndarray, weights: np.ndarray) -> np.ndarray:
+    """Output biases (keep, clone, split) = (0, d_clone, d_split) chosen on the validation set so
+    that the network proposes clones and splits as often as the (sample-weighted) labels hold
+    them. Trained on rare densify labels, the plain argmax under-proposes them."""
+    total = weights.sum()
+    target = {c: float(weights[labels == c].sum() / total) for c in (1, 2)}
+    biases = np.zeros(3)
+    grid = np.linspace(-8.0, 8.0, 321)
+    for _ in range(3):
+        for c in (2, 1):
+            best, best_error = 0.0, np.inf
+            for value in grid:
+                trial = biases.copy()
+                trial[c] = value
+                predicted = np.argmax(logits + trial[None, :], axis=1)
+                error = abs(float(weights[predicted == c].sum() / total) - target[c])
+                if error < best_error - 1e-12:
+                    best, best_error = value, error
+            biases[c] = best
+    return biases
 
 
 def time_to_psnr(curve, target):
@@ -213,6 +241,8 @@ def main() -> int:
                              "and at least this opaque")
     parser.add_argument("--colour-noise", type=float, default=0.05,
                         help="per-Gaussian colour noise of the truth scenes (their fine texture)")
+    parser.add_argument("--no-calibration", action="store_true",
+                        help="keep the trained output biases (plain argmax) instead of matching the label rates")
     parser.add_argument("--quick", action="store_true", help="small sizes for a functional check")
     parser.add_argument("--out", type=str, default="")
     args = parser.parse_args()
@@ -243,7 +273,7 @@ def main() -> int:
     represented = float(weights.sum())
     log(f"samples: {features.shape[0]} stored, {represented:.0f} represented, labels {np.bincount(labels, minlength=3)}")
     network_params, training = train_network(context, features, labels, weights, args.epochs, args.batch,
-                                              args.learning_rate, seed=7)
+                                              args.learning_rate, seed=7, calibrate=not args.no_calibration)
     training["represented_samples"] = represented
     log(f"trained: validation accuracy {training['validation_accuracy']:.4f}")
     del features, labels, weights, outcome_log
