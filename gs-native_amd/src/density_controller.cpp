@@ -222,18 +222,37 @@ bool DensityController::ControlDensity(GaussianBuffers& gaussians, float scene_e
                        observation_counts_, feature_state_, feature_capacity_, gradient_threshold, split_threshold,
                        scene_extent, features_),
                 "gsn_density_features") ||
-            !proposal_->Predict(features_, count, logits_) ||
+            !proposal_->Predict(features_, count, logits_)) {
+            return false;
+        }
+    }
+    auto rule_counts = [&](int* out) {
+        return Check(gsn_density_count(context, parameters, count, gradient_sums_, absolute_gradient_sums_,
+                         observation_counts_, gradient_threshold, split_threshold, min_opacity, 0.1f * scene_extent,
+                         max_screen_size > 0, absolute_gradient_threshold, density_counts_, density_offsets_, out),
+            "gsn_density_count");
+    };
+    if (proposed) {
+        // Growth guard: the rule's count first, then the proposal; a proposal adding more than
+        // twice the rule's Gaussians (and more than 1% of the scene) is not followed.
+        int rule_totals[4] = {0, 0, 0, 0};
+        if (!rule_counts(rule_totals) ||
             !Check(gsn_density_count_proposed(context, parameters, count, gradient_sums_, absolute_gradient_sums_,
                        observation_counts_, logits_, split_threshold, min_opacity, 0.1f * scene_extent,
                        max_screen_size > 0, density_counts_, density_offsets_, totals),
                 "gsn_density_count_proposed")) {
             return false;
         }
+        int rule_added = rule_totals[0] + rule_totals[1] + 2 * rule_totals[3] - count;
+        int proposed_added = totals[0] + totals[1] + 2 * totals[3] - count;
+        if (proposed_added > std::max(2 * std::max(rule_added, 0), count / 100)) {
+            ++proposal_fallbacks_;
+            if (!rule_counts(totals)) {
+                return false;
+            }
+        }
     }
-    else if (!Check(gsn_density_count(context, parameters, count, gradient_sums_, absolute_gradient_sums_,
-                        observation_counts_, gradient_threshold, split_threshold, min_opacity, 0.1f * scene_extent,
-                        max_screen_size > 0, absolute_gradient_threshold, density_counts_, density_offsets_, totals),
-                 "gsn_density_count")) {
+    else if (!rule_counts(totals)) {
         return false;
     }
     int required = totals[0] + totals[1] + 2 * totals[3];

@@ -79,6 +79,11 @@ public:
         proposal_ = network;
     }
 
+    // Density steps in which the proposal was not followed because it added more than twice the
+    // Gaussians the plain rule would (and more than 1% of the scene): the rule decided instead.
+    int GetProposalFallbacks() const {
+        return proposal_fallbacks_;
+    }
     const ProposalNetwork* GetProposal() const {
         return proposal_;
     }
@@ -108,6 +113,7 @@ private:
     int* density_offsets_ = nullptr;
     int* control_ = nullptr;
     const ProposalNetwork* proposal_ = nullptr;
+    int proposal_fallbacks_ = 0;
     float* feature_state_ = nullptr;  // [6][capacity_]
     float* features_ = nullptr;       // [capacity_][16]
     float* logits_ = nullptr;         // [capacity_][3]

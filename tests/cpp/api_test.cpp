@@ -363,8 +363,34 @@ void ProposalNetworkDrivesDensityControl() {
         else {
             EXPECT(changes[0] == 2 && changes[1] == 1 && buffers.gaussian_count_ == 4);
         }
+        EXPECT(controller.GetProposalFallbacks() == 0);
         (void)hipFree(buffers.gaussians_);
     }
+
+    // Growth guard: with gradients below the threshold the rule adds nothing, so a network that
+    // splits everything is not followed and the step matches the rule (one prune, no additions).
+    std::vector<float> split(static_cast<size_t>(n), 0.0f);
+    split[n - 1] = 1.0f;
+    ProposalNetwork split_all;
+    EXPECT(split_all.Load(split));
+    auto gaussians = MakeGaussians(3);
+    gaussians[1].opacity_logit_ = std::log(0.001f / 0.999f);
+    DeviceVector<float> original(PackGaussians(gaussians));
+    DensityController controller;
+    EXPECT(controller.Initialize(3, 3));
+    controller.SetProposal(&split_all);
+    DeviceVector<float> statistics(std::vector<float>{0.0f, 0.0f, 0.0f, 1.0f, 1.0f, 1.0f});
+    DeviceVector<float> absolute(std::vector<float>{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f});
+    EXPECT(controller.AccumulateStatistics(statistics.data, absolute.data, 3));
+    float* copy = nullptr;
+    EXPECT(hipMalloc(reinterpret_cast<void**>(&copy), original.count * sizeof(float)) == hipSuccess);
+    EXPECT(hipMemcpy(copy, original.data, original.count * sizeof(float), hipMemcpyDeviceToDevice) == hipSuccess);
+    GaussianBuffers buffers{copy, 3};
+    int changes[2] = {};
+    EXPECT(controller.ControlDensity(buffers, 1.0f, 0.15f, 0.01f, 0.005f, 0.0f, 42, changes));
+    EXPECT(changes[0] == 0 && changes[1] == 1 && buffers.gaussian_count_ == 2);
+    EXPECT(controller.GetProposalFallbacks() == 1);
+    (void)hipFree(buffers.gaussians_);
 }
 
 // solver_test.cu:320-336: opacity history reset -> first-step magnitude at step 2.
