@@ -79,16 +79,20 @@ def profiled_value(key: str, stage: str):
     return None
 
 
-def build_workload(gsn, context, gaussians, width, height, cameras, seed=0):
+def build_workload(gsn, context, gaussians, width, height, cameras, seed=0, scene="blobs", camera_radius=5.0,
+                   hemisphere=False):
     scenes_spec = importlib.util.spec_from_file_location("gs_native_amd.scenes", REPO / "gs-native_amd" / "scenes.py")
     scenes = importlib.util.module_from_spec(scenes_spec)
     scenes_spec.loader.exec_module(scenes)
-    truth = scenes.blob_scene(gaussians, seed=seed, sh_degree=3)
+    if scene == "shell":
+        truth = scenes.shell_plane_scene(gaussians, seed=seed, sh_degree=3)
+    else:
+        truth = scenes.blob_scene(gaussians, seed=seed, sh_degree=3)
     _, mean_dist2 = gsn.knn3(context, truth[0:3].T.copy())
     truth[3:6] = scenes.scales_from_mean_dist2(mean_dist2)[None, :]
     params = scenes.perturb(truth, seed + 1, 0.01)
     focal = scenes.fov_focal(width, 60.0)
-    poses = scenes.sphere_cameras(cameras, 5.0, seed + 2)
+    poses = scenes.sphere_cameras(cameras, camera_radius, seed + 2, hemisphere=hemisphere)
     cams = [gsn.Camera.make(width, height, focal, focal, 0.5 * (width - 1), 0.5 * (height - 1), R, t) for R, t in poses]
     centres = np.stack([-R.T @ t for R, t in poses])
     extent = 1.1 * float(np.linalg.norm(centres - centres.mean(0), axis=1).max())
@@ -146,6 +150,10 @@ def main() -> int:
     parser.add_argument("--height", type=int, default=1080)
     parser.add_argument("--cameras", type=int, default=300)
     parser.add_argument("--start-iteration", type=int, default=3000)
+    parser.add_argument("--scene", choices=["blobs", "shell"], default="blobs",
+                        help="blobs: seeded 64-blob mixture; shell: radius-2 shell plus ground plane")
+    parser.add_argument("--camera-radius", type=float, default=5.0)
+    parser.add_argument("--hemisphere", action="store_true", help="cameras on the upper hemisphere only")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--calibration-copy-mib", type=int, default=0,
                         help="after the timed steps, run one device-to-device copy of this many MiB (a plain copy "
@@ -162,8 +170,9 @@ def main() -> int:
     context = gsn.Context()
     log(f"device {gsn.device_name()}")
     log(f"building workload: {args.gaussians} Gaussians, {args.width}x{args.height}, {args.cameras} cameras")
-    params, cameras, targets, extent, visible, records = build_workload(gsn, context, args.gaussians, args.width,
-                                                                         args.height, args.cameras)
+    params, cameras, targets, extent, visible, records = build_workload(
+        gsn, context, args.gaussians, args.width, args.height, args.cameras, scene=args.scene,
+        camera_radius=args.camera_radius, hemisphere=args.hemisphere)
     log(f"workload ready: mean visible {np.mean(visible):.0f}, mean records {np.mean(records):.0f}")
     trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=args.start_iteration)
     for _ in range(args.warmup):
@@ -218,8 +227,11 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.gaussians // 1_000_000}M-Gaussian 64-blob scene, {args.width}x{args.height}, SH degree 3, "
-                        f"{args.cameras} cameras, training iterations {args.start_iteration + args.warmup + 1}-"
+            "workload": f"{args.gaussians:,}-Gaussian "
+                        f"{'shell + ground-plane' if args.scene == 'shell' else '64-blob'} scene, "
+                        f"{args.width}x{args.height}, SH degree 3, {args.cameras} cameras "
+                        f"({'hemisphere' if args.hemisphere else 'sphere'}, radius {args.camera_radius:g}), "
+                        f"training iterations {args.start_iteration + args.warmup + 1}-"
                         f"{args.start_iteration + args.warmup + args.steps} of the reference schedule",
             "gaussians_start": start_count,
             "gaussians_end": trainer.count,

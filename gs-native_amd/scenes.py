@@ -147,6 +147,34 @@ def blob_scene(count: int, seed: int, blobs: int = 64, sh_degree: int = 3, colou
     return p
 
 
+def shell_plane_scene(count: int, seed: int, sh_degree: int = 3, plane_fraction: float = 0.3) -> np.ndarray:
+    """Gaussians on a radius-2 spherical shell (0.05 thick) plus a ground plane at y = 2 (y points
+    down in the camera convention of sphere_cameras(hemisphere=True), so the cameras look at the
+    shell from above the plane), x and z in [-4, 4]: the 500,000-Gaussian GPU configuration.
+    Log-scales are set later from 3-NN distances; random rotations, opacity logit N(0, 1), SH DC
+    from uniform colours, higher bands N(0, 0.05)."""
+    rng = np.random.default_rng(seed)
+    planar = int(round(count * plane_fraction))
+    shell = count - planar
+    directions = rng.normal(size=(shell, 3))
+    directions /= np.linalg.norm(directions, axis=1, keepdims=True)
+    radii = 2.0 + rng.uniform(-0.025, 0.025, size=(shell, 1))
+    ground = np.stack([rng.uniform(-4.0, 4.0, planar), 2.0 + rng.normal(0.0, 0.01, planar),
+                       rng.uniform(-4.0, 4.0, planar)], axis=1)
+    positions = np.concatenate([directions * radii, ground])
+    p = empty_params(count)
+    p[0:3] = positions.T
+    p[3:6] = -4.0
+    p[6:9] = random_rotations(rng, count).T
+    p[9] = rng.normal(0.0, 1.0, size=count)
+    rgb = rng.uniform(0.0, 1.0, size=(count, 3))
+    for c in range(3):
+        p[10 + 16 * c] = (rgb[:, c] - 0.5) / SH_C0
+        if sh_degree > 0:
+            p[11 + 16 * c: 26 + 16 * c] = rng.normal(0.0, 0.05, size=(15, count))
+    return p
+
+
 def scales_from_mean_dist2(mean_dist2: np.ndarray) -> np.ndarray:
     """gaussian_splatting.cpp:103-113: log-scale = 0.5 * log(max(d2, 1e-7))."""
     return (0.5 * np.log(np.maximum(mean_dist2.astype(np.float32), np.float32(1e-7)))).astype(np.float32)
