@@ -100,74 +100,193 @@ __device__ __forceinline__ void ssim_moments(SsimTile& s, const Window& win, con
     }
 }
 
+// Register blocking: each thread filters a run of adjacent outputs from one window of values read
+// from LDS once (11-tap windows overlap by 10), which cuts LDS reads 3-4x against one thread per
+// output; every output still sums its taps in the same order, so results are unchanged.
+constexpr int kRunH = 6;   // horizontal moments: outputs per thread (kMw = 42 = 7 runs)
+constexpr int kRunV = 7;   // vertical moments: rows per thread (kMh = 26: runs of 7, 7, 7, 5)
+constexpr int kRunA = 4;   // horizontal adjoint: outputs per thread (kTw = 32 = 8 runs)
+constexpr int kRunO = 2;   // vertical adjoint: rows per thread (kTh = 16 = 8 runs)
+static_assert(kMw % kRunH == 0 && kTw % kRunA == 0 && kTh % kRunO == 0, "runs must tile the rows");
+
+__device__ __forceinline__ void ssim_coefficients(const float (&m)[5], bool inside, float& a, float& b, float& c) {
+    a = 0.0f;
+    b = 0.0f;
+    c = 0.0f;
+    if (inside) {
+        float mu_a = m[0], mu_b = m[1];
+        float inverse_d = 1.0f / (mu_a * mu_a + mu_b * mu_b + 0.0001f);
+        float inverse_e = 1.0f / (m[2] - mu_a * mu_a + m[3] - mu_b * mu_b + 0.0009f);
+        float luminance = (2.0f * mu_a * mu_b + 0.0001f) * inverse_d;
+        float contrast = (2.0f * (m[4] - mu_a * mu_b) + 0.0009f) * inverse_e;
+        float mean_derivative = 2.0f * (mu_b - mu_a * luminance) * inverse_d * contrast;
+        b = 2.0f * luminance * inverse_e;
+        c = -luminance * contrast * inverse_e;
+        a = mean_derivative - mu_b * b - 2.0f * mu_a * c;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void image_gradients_kernel(const float* __restrict__ image,
     const uint8_t* __restrict__ target, float* __restrict__ gradients, int width, int height, Window win) {
     __shared__ SsimTile s;
     const int channel = blockIdx.z;
     const int origin_x = blockIdx.x * kTw;
     const int origin_y = blockIdx.y * kTh;
-    // Coefficient maps reuse the input region once the moments exist; the horizontal adjoint
-    // reuses the moment region.
-    float* coef = s.in;  // [3][kMh][kMw]
-    ssim_moments(s, win, image, target, width, height, channel, origin_x, origin_y,
-        [&](int my, int mx, const float (&m)[5], bool inside) {
-            float a = 0.0f, b = 0.0f, c = 0.0f;
-            if (inside) {
-                float mu_a = m[0], mu_b = m[1];
-                float inverse_d = 1.0f / (mu_a * mu_a + mu_b * mu_b + 0.0001f);
-                float inverse_e = 1.0f / (m[2] - mu_a * mu_a + m[3] - mu_b * mu_b + 0.0009f);
-                float luminance = (2.0f * mu_a * mu_b + 0.0001f) * inverse_d;
-                float contrast = (2.0f * (m[4] - mu_a * mu_b) + 0.0009f) * inverse_e;
-                float mean_derivative = 2.0f * (mu_b - mu_a * luminance) * inverse_d * contrast;
-                b = 2.0f * luminance * inverse_e;
-                c = -luminance * contrast * inverse_e;
-                a = mean_derivative - mu_b * b - 2.0f * mu_a * c;
-            }
-            // All reads of s.in finished before the barrier that precedes this stage.
-            coef[0 * kMh * kMw + my * kMw + mx] = a;
-            coef[1 * kMh * kMw + my * kMw + mx] = b;
-            coef[2 * kMh * kMw + my * kMw + mx] = c;
-        });
-    __syncthreads();
-    float* adjoint = s.h;  // [3][kMh][kTw]
     const int t = threadIdx.x;
-    for (int idx = t; idx < kMh * kTw; idx += kThreads) {
-        int my = idx / kTw, ox = idx - my * kTw;
-        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-#pragma unroll
-        for (int dx = 0; dx < 11; ++dx) {
-            float w = win.w[dx];
-            f0 += w * coef[0 * kMh * kMw + my * kMw + ox + dx];
-            f1 += w * coef[1 * kMh * kMw + my * kMw + ox + dx];
-            f2 += w * coef[2 * kMh * kMw + my * kMw + ox + dx];
-        }
-        adjoint[0 * kMh * kTw + idx] = f0;
-        adjoint[1 * kMh * kTw + idx] = f1;
-        adjoint[2 * kMh * kTw + idx] = f2;
+    // 1. Image and target with a 10-pixel halo (zero outside the image).
+    for (int idx = t; idx < kIh * kIw; idx += kThreads) {
+        int iy = idx / kIw, ix = idx - iy * kIw;
+        int gx = origin_x - 10 + ix, gy = origin_y - 10 + iy;
+        bool valid = gx >= 0 && gy >= 0 && gx < width && gy < height;
+        int64_t index = (static_cast<int64_t>(gy) * width + gx) * 3 + channel;
+        s.in[idx] = valid ? image[index] : 0.0f;
+        s.in[kIh * kIw + idx] = valid ? static_cast<float>(target[index]) * (1.0f / 255.0f) : 0.0f;
     }
     __syncthreads();
-    const float normalization = 1.0f / (3.0f * static_cast<float>(width) * static_cast<float>(height));
-    for (int idx = t; idx < kTh * kTw; idx += kThreads) {
-        int oy = idx / kTw, ox = idx - oy * kTw;
-        int x = origin_x + ox, y = origin_y + oy;
-        if (x >= width || y >= height) {
-            continue;
-        }
-        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+    // 2. Horizontal pass of the five moments, kRunH adjacent columns per thread.
+    for (int item = t; item < kIh * (kMw / kRunH); item += kThreads) {
+        const int iy = item / (kMw / kRunH);
+        const int column = (item - iy * (kMw / kRunH)) * kRunH;
+        float a[kRunH + 10], b[kRunH + 10];
 #pragma unroll
-        for (int dy = 0; dy < 11; ++dy) {
-            float w = win.w[dy];
-            f0 += w * adjoint[0 * kMh * kTw + (oy + dy) * kTw + ox];
-            f1 += w * adjoint[1 * kMh * kTw + (oy + dy) * kTw + ox];
-            f2 += w * adjoint[2 * kMh * kTw + (oy + dy) * kTw + ox];
+        for (int j = 0; j < kRunH + 10; ++j) {
+            a[j] = s.in[iy * kIw + column + j];
+            b[j] = s.in[kIh * kIw + iy * kIw + column + j];
         }
-        int64_t pixel = (static_cast<int64_t>(y) * width + x) * 3 + channel;
-        float render = image[pixel];
-        float reference = static_cast<float>(target[pixel]) * (1.0f / 255.0f);
-        float residual = render - reference;
-        float l1_gradient = static_cast<float>(residual > 0.0f) - static_cast<float>(residual < 0.0f);
-        float ssim_gradient = -(f0 + reference * f1 + 2.0f * render * f2);
-        gradients[pixel] = (0.8f * l1_gradient + 0.2f * ssim_gradient) * normalization;
+#pragma unroll
+        for (int g = 0; g < kRunH; ++g) {
+            float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f;
+#pragma unroll
+            for (int dx = 0; dx < 11; ++dx) {
+                float w = win.w[dx];
+                float av = a[g + dx];
+                float bv = b[g + dx];
+                m0 += w * av;
+                m1 += w * bv;
+                m2 += w * av * av;
+                m3 += w * bv * bv;
+                m4 += w * av * bv;
+            }
+            const int idx = iy * kMw + column + g;
+            s.h[0 * kIh * kMw + idx] = m0;
+            s.h[1 * kIh * kMw + idx] = m1;
+            s.h[2 * kIh * kMw + idx] = m2;
+            s.h[3 * kIh * kMw + idx] = m3;
+            s.h[4 * kIh * kMw + idx] = m4;
+        }
+    }
+    __syncthreads();
+    // 3. Vertical pass (kRunV rows per thread) and the per-position SSIM derivative coefficients
+    //    (a, b, c); the coefficient maps reuse the input region, whose reads all ended above.
+    float* coef = s.in;  // [3][kMh][kMw]
+    constexpr int kRunsV = (kMh + kRunV - 1) / kRunV;
+    for (int item = t; item < kMw * kRunsV; item += kThreads) {
+        const int mx = item % kMw;
+        const int row0 = (item / kMw) * kRunV;
+        const int rows = min(kRunV, kMh - row0);
+        float m[kRunV][5];
+#pragma unroll
+        for (int g = 0; g < kRunV; ++g) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                m[g][k] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            float v[kRunV + 10];
+#pragma unroll
+            for (int j = 0; j < kRunV + 10; ++j) {
+                const int row = min(row0 + j, kIh - 1);  // rows past the region feed only unused outputs
+                v[j] = s.h[k * kIh * kMw + row * kMw + mx];
+            }
+#pragma unroll
+            for (int dy = 0; dy < 11; ++dy) {
+                float w = win.w[dy];
+#pragma unroll
+                for (int g = 0; g < kRunV; ++g) {
+                    m[g][k] += w * v[g + dy];
+                }
+            }
+        }
+        const int hx = origin_x - 5 + mx;
+#pragma unroll
+        for (int g = 0; g < kRunV; ++g) {
+            if (g < rows) {
+                const int my = row0 + g;
+                const int hy = origin_y - 5 + my;
+                const bool inside = hx >= 0 && hy >= 0 && hx < width && hy < height;
+                float ca, cb, cc;
+                ssim_coefficients(m[g], inside, ca, cb, cc);
+                coef[0 * kMh * kMw + my * kMw + mx] = ca;
+                coef[1 * kMh * kMw + my * kMw + mx] = cb;
+                coef[2 * kMh * kMw + my * kMw + mx] = cc;
+            }
+        }
+    }
+    __syncthreads();
+    // 4. Horizontal adjoint of the three coefficient maps, kRunA adjacent columns per thread.
+    float* adjoint = s.h;  // [3][kMh][kTw]
+    for (int item = t; item < kMh * (kTw / kRunA); item += kThreads) {
+        const int my = item / (kTw / kRunA);
+        const int column = (item - my * (kTw / kRunA)) * kRunA;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            float v[kRunA + 10];
+#pragma unroll
+            for (int j = 0; j < kRunA + 10; ++j) {
+                v[j] = coef[f * kMh * kMw + my * kMw + column + j];
+            }
+#pragma unroll
+            for (int g = 0; g < kRunA; ++g) {
+                float sum = 0.0f;
+#pragma unroll
+                for (int dx = 0; dx < 11; ++dx) {
+                    sum += win.w[dx] * v[g + dx];
+                }
+                adjoint[f * kMh * kTw + my * kTw + column + g] = sum;
+            }
+        }
+    }
+    __syncthreads();
+    // 5. Vertical adjoint (kRunO rows per thread) and dL/dI.
+    const float normalization = 1.0f / (3.0f * static_cast<float>(width) * static_cast<float>(height));
+    for (int item = t; item < kTw * (kTh / kRunO); item += kThreads) {
+        const int ox = item % kTw;
+        const int row0 = (item / kTw) * kRunO;
+        float f[kRunO][3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            float v[kRunO + 10];
+#pragma unroll
+            for (int j = 0; j < kRunO + 10; ++j) {
+                v[j] = adjoint[m * kMh * kTw + (row0 + j) * kTw + ox];
+            }
+#pragma unroll
+            for (int g = 0; g < kRunO; ++g) {
+                float sum = 0.0f;
+#pragma unroll
+                for (int dy = 0; dy < 11; ++dy) {
+                    sum += win.w[dy] * v[g + dy];
+                }
+                f[g][m] = sum;
+            }
+        }
+        const int x = origin_x + ox;
+#pragma unroll
+        for (int g = 0; g < kRunO; ++g) {
+            const int y = origin_y + row0 + g;
+            if (x >= width || y >= height) {
+                continue;
+            }
+            int64_t pixel = (static_cast<int64_t>(y) * width + x) * 3 + channel;
+            float render = image[pixel];
+            float reference = static_cast<float>(target[pixel]) * (1.0f / 255.0f);
+            float residual = render - reference;
+            float l1_gradient = static_cast<float>(residual > 0.0f) - static_cast<float>(residual < 0.0f);
+            float ssim_gradient = -(f[g][0] + reference * f[g][1] + 2.0f * render * f[g][2]);
+            gradients[pixel] = (0.8f * l1_gradient + 0.2f * ssim_gradient) * normalization;
+        }
     }
 }
 
