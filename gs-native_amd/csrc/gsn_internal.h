@@ -35,6 +35,7 @@ enum ScratchSlot {
     kSlotSplats,
     kSlotRecordFlags,
     kSlotTileOrder,
+    kSlotEmitOwners,
     kSlotCount
 };
 
@@ -112,8 +113,15 @@ int exclusive_scan_int4(gsn_context* context, const Int4* input, Int4* output, i
 // (radix_sort.hip). Input buffers are consumed. The result is written to keys_out/values_out
 // (either may be the same buffer as an input only if it is not referenced by a later pass;
 // callers pass distinct buffers). keys_out may be null when only values are needed.
+// With first_histogram_ready the caller has already written the first pass's digit counts into
+// radix_histogram_buffer(context, n): entry [bin * tiles + tile] counts the elements of
+// [tile * kRadixTile, tile * kRadixTile + kRadixTile) whose low radix_first_digit_bits(bits) key
+// bits equal bin, for every bin below 1 << radix_first_digit_bits(bits).
+constexpr int kRadixTile = 4096;
+int* radix_histogram_buffer(gsn_context* context, int64_t n);
+int radix_first_digit_bits(int bits);
 int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int64_t n,
-    int bits, uint32_t* keys_out, uint32_t* values_out);
+    int bits, uint32_t* keys_out, uint32_t* values_out, bool first_histogram_ready = false);
 
 // Floats per (tile, Gaussian) backward partial: dmean2D(2), dSigma2D(3), dlogit, dcolor(3),
 // |dmean2D|(2), pad.

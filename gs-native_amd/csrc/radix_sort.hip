@@ -15,7 +15,8 @@ namespace {
 
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
+constexpr int kSortTile = kSortThreads * kSortItems;
+static_assert(kSortTile == kRadixTile, "histogram columns cover kRadixTile elements");
 constexpr int kWaves = kSortThreads / 64;
 constexpr int kWaveChunk = kSortTile / kWaves;        // 1024
 
@@ -154,8 +155,19 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
 
 } // namespace
 
+int* radix_histogram_buffer(gsn_context* context, int64_t n) {
+    int64_t tiles = (n + kSortTile - 1) / kSortTile;
+    return static_cast<int*>(scratch(context, kSlotSortHistogram, static_cast<size_t>(tiles) * 256 * sizeof(int)));
+}
+
+int radix_first_digit_bits(int bits) {
+    bits = bits < 1 ? 1 : (bits > 32 ? 32 : bits);
+    int passes = (bits + 7) / 8;
+    return (bits + passes - 1) / passes;
+}
+
 int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int64_t n, int bits,
-    uint32_t* keys_out, uint32_t* values_out) {
+    uint32_t* keys_out, uint32_t* values_out, bool first_histogram_ready) {
     if (n <= 0) {
         return GSN_OK;
     }
@@ -173,7 +185,7 @@ int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int
     size_t elements = static_cast<size_t>(n);
     uint32_t* scratch_keys = static_cast<uint32_t*>(scratch(context, kSlotSortKeysA, elements * sizeof(uint32_t)));
     uint32_t* scratch_values = static_cast<uint32_t*>(scratch(context, kSlotSortValuesA, elements * sizeof(uint32_t)));
-    int* histogram = static_cast<int*>(scratch(context, kSlotSortHistogram, static_cast<size_t>(tiles) * 256 * sizeof(int)));
+    int* histogram = radix_histogram_buffer(context, n);
     if (!scratch_keys || !scratch_values || !histogram) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
@@ -198,8 +210,11 @@ int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int
         bool to_scratch = (pass % 2) == 0;
         uint32_t* dst_keys = last ? final_keys : (to_scratch ? scratch_keys : keys);
         uint32_t* dst_values = last ? values_out : (to_scratch ? scratch_values : values);
-        radix_histogram_kernel<<<tiles, kSortThreads, 0, context->stream>>>(src_keys, n, shift, bins, tiles, histogram);
-        GSN_TRY(check_launch("radix_histogram_kernel"));
+        if (pass > 0 || !first_histogram_ready) {
+            radix_histogram_kernel<<<tiles, kSortThreads, 0, context->stream>>>(src_keys, n, shift, bins, tiles,
+                histogram);
+            GSN_TRY(check_launch("radix_histogram_kernel"));
+        }
         GSN_TRY(exclusive_scan_int(context, histogram, histogram, static_cast<int64_t>(bins) * tiles));
         radix_scatter_kernel<<<tiles, kSortThreads, 0, context->stream>>>(src_keys, src_values, n, shift, digit_bits, tiles,
             histogram, dst_keys, dst_values);

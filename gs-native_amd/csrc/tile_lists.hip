@@ -39,31 +39,175 @@ __global__ __launch_bounds__(256) void depth_order_counts_kernel(const uint32_t*
     counts[k] = record_offsets[id + 1] - record_offsets[id];
 }
 
-// Eight lanes per Gaussian enumerate its rectangle row-major.
-constexpr int kEmitLanes = 8;
+// Records are emitted in blocks of kRadixTile consecutive positions, one workgroup per block, so
+// every workgroup writes the same amount whatever the footprints (a near-camera Gaussian can own
+// thousands of records, a distant one a single record). A workgroup first needs the Gaussians
+// (in depth order) that own its positions: emit_owners_kernel stores, for every block, the
+// depth-order index of the Gaussian owning the block's first record, and in entry `blocks` the
+// owner of the last record, so block b's owners are the contiguous range [owners[b], owners[b+1]].
+constexpr int kEmitThreads = 512;
+constexpr int kEmitItems = kRadixTile / kEmitThreads;
+constexpr int kEmitOwners = kRadixTile + 1;
 
-__global__ __launch_bounds__(256) void emit_records_kernel(const uint32_t* sorted_ids, const int* depth_offsets,
-    const int4* tile_rects, uint32_t* record_tiles, uint32_t* record_ids, int count, int tile_count_x,
-    int record_count) {
-    int k = (blockIdx.x * blockDim.x + threadIdx.x) / kEmitLanes;
-    int lane = threadIdx.x % kEmitLanes;
+__global__ __launch_bounds__(256) void emit_owners_kernel(const int* depth_offsets, int count, int record_count,
+    int blocks, int* owners) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) {
         return;
     }
-    int begin = depth_offsets[k];
-    int total = depth_offsets[k + 1] - begin;
-    if (total == 0) {
+    int64_t begin = max(depth_offsets[k], 0);
+    int64_t end = min(depth_offsets[k + 1], record_count);
+    if (end <= begin) {
         return;
     }
-    uint32_t id = sorted_ids[k];
-    int4 rect = tile_rects[id];
-    int width = rect.z - rect.x;
-    // record_count bounds the writes even if record_offsets disagree with it.
-    for (int local = lane; local < total && begin + local < record_count; local += kEmitLanes) {
-        int row = local / width;
-        int column = local - row * width;
-        record_tiles[begin + local] = static_cast<uint32_t>((rect.y + row) * tile_count_x + rect.x + column);
-        record_ids[begin + local] = id;
+    for (int64_t b = (begin + kRadixTile - 1) / kRadixTile; b < blocks && b * kRadixTile < end; ++b) {
+        owners[b] = k;
+    }
+    if (begin <= record_count - 1 && record_count - 1 < end) {
+        owners[blocks] = k;
+    }
+}
+
+// One workgroup per block of kRadixTile records. Owner data (record start, id, rectangle) is
+// staged in LDS, every position is mapped to its owner by a max-scan over the positions where
+// owners start, and consecutive lanes then write consecutive records: (row-major tile inside the
+// owner's rectangle, Gaussian id). The workgroup also counts the low `digit_bits` bits of its
+// tile ids, which is the first pass's histogram column of the radix sort that follows (the
+// block is exactly one radix tile).
+__global__ __launch_bounds__(kEmitThreads) void emit_records_kernel(const uint32_t* __restrict__ sorted_ids,
+    const int* __restrict__ depth_offsets, const int* __restrict__ owners, const int4* __restrict__ tile_rects,
+    uint32_t* __restrict__ record_tiles, uint32_t* __restrict__ record_ids, int count, int tile_count_x,
+    int record_count, int blocks, int digit_bits, int* __restrict__ histogram) {
+    __shared__ int s_owner[kRadixTile];
+    __shared__ int s_start[kEmitOwners];
+    __shared__ uint32_t s_id[kEmitOwners];
+    __shared__ uint32_t s_corner[kEmitOwners];  // x0 | y0 << 16
+    __shared__ uint16_t s_width[kEmitOwners];
+    __shared__ int s_digits[256];
+    __shared__ int s_wave_max[kEmitThreads / 64];
+    const int t = threadIdx.x;
+    const int block = blockIdx.x;
+    const int64_t base = static_cast<int64_t>(block) * kRadixTile;
+    const int valid = static_cast<int>(min(static_cast<int64_t>(kRadixTile), record_count - base));
+    const uint32_t digit_mask = (1u << digit_bits) - 1u;
+    if (t < 256) {
+        s_digits[t] = 0;
+    }
+    const int first = min(max(owners[block], 0), count - 1);
+    const int last = min(max(owners[block + 1], first), count - 1);
+    const int owner_count = last - first + 1;
+    if (owner_count <= kEmitOwners) {
+        for (int i = t; i < kRadixTile; i += kEmitThreads) {
+            s_owner[i] = 0;
+        }
+        __syncthreads();
+        for (int o = t; o < owner_count; o += kEmitThreads) {
+            int k = first + o;
+            int start = depth_offsets[k];
+            uint32_t id = min(sorted_ids[k], static_cast<uint32_t>(count - 1));
+            int4 rect = tile_rects[id];
+            s_start[o] = start;
+            s_id[o] = id;
+            s_corner[o] = static_cast<uint32_t>(rect.x & 0xffff) | (static_cast<uint32_t>(rect.y) << 16);
+            s_width[o] = static_cast<uint16_t>(max(rect.z - rect.x, 1));
+            int64_t position = static_cast<int64_t>(start) - base;
+            if (position < kRadixTile) {
+                // The block's first owner starts at or before the block; later owners start inside
+                // it, and of several owners starting at one position (owners without records) the
+                // last is the one whose records follow.
+                atomicMax(&s_owner[max(position, int64_t{0})], o);
+            }
+        }
+        __syncthreads();
+        // Inclusive max-scan over the positions: thread t owns positions [8t, 8t + 8).
+        int v[kEmitItems];
+        const int4 lo = *reinterpret_cast<const int4*>(&s_owner[kEmitItems * t]);
+        const int4 hi = *reinterpret_cast<const int4*>(&s_owner[kEmitItems * t + 4]);
+        v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
+        v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
+#pragma unroll
+        for (int i = 1; i < kEmitItems; ++i) {
+            v[i] = max(v[i], v[i - 1]);
+        }
+        const int lane = t & 63;
+        const int wave = t >> 6;
+        int scan = v[kEmitItems - 1];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int other = __shfl_up(scan, d, 64);
+            if (lane >= d) {
+                scan = max(scan, other);
+            }
+        }
+        if (lane == 63) {
+            s_wave_max[wave] = scan;
+        }
+        int carry = __shfl_up(scan, 1, 64);
+        if (lane == 0) {
+            carry = 0;
+        }
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) {
+            carry = max(carry, s_wave_max[w]);
+        }
+        int4 out_lo, out_hi;
+        out_lo.x = max(v[0], carry); out_lo.y = max(v[1], carry); out_lo.z = max(v[2], carry); out_lo.w = max(v[3], carry);
+        out_hi.x = max(v[4], carry); out_hi.y = max(v[5], carry); out_hi.z = max(v[6], carry); out_hi.w = max(v[7], carry);
+        *reinterpret_cast<int4*>(&s_owner[kEmitItems * t]) = out_lo;
+        *reinterpret_cast<int4*>(&s_owner[kEmitItems * t + 4]) = out_hi;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kEmitItems; ++j) {
+            int p = j * kEmitThreads + t;
+            if (p < valid) {
+                int o = s_owner[p];
+                int local = static_cast<int>(base + p - s_start[o]);
+                int width = s_width[o];
+                int row = local / width;
+                int column = local - row * width;
+                uint32_t corner = s_corner[o];
+                uint32_t tile = static_cast<uint32_t>(((corner >> 16) + row) * tile_count_x + (corner & 0xffffu) + column);
+                record_tiles[base + p] = tile;
+                record_ids[base + p] = s_id[o];
+                atomicAdd(&s_digits[tile & digit_mask], 1);
+            }
+        }
+    }
+    else {
+        // Only reachable when Gaussians without records sit between ones with records (the
+        // projection never produces that): find each position's owner by binary search.
+        __syncthreads();
+        for (int j = 0; j < kEmitItems; ++j) {
+            int p = j * kEmitThreads + t;
+            if (p >= valid) {
+                continue;
+            }
+            int r = static_cast<int>(base + p);
+            int low = 0, high = count;
+            while (high - low > 1) {
+                int middle = (low + high) >> 1;
+                if (depth_offsets[middle] <= r) {
+                    low = middle;
+                }
+                else {
+                    high = middle;
+                }
+            }
+            uint32_t id = min(sorted_ids[low], static_cast<uint32_t>(count - 1));
+            int4 rect = tile_rects[id];
+            int width = max(rect.z - rect.x, 1);
+            int local = r - depth_offsets[low];
+            int row = local / width;
+            int column = local - row * width;
+            uint32_t tile = static_cast<uint32_t>((rect.y + row) * tile_count_x + rect.x + column);
+            record_tiles[r] = tile;
+            record_ids[r] = id;
+            atomicAdd(&s_digits[tile & digit_mask], 1);
+        }
+    }
+    __syncthreads();
+    if (t < (1 << digit_bits)) {
+        histogram[static_cast<int64_t>(t) * blocks + block] = s_digits[t];
     }
 }
 
@@ -138,6 +282,9 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
     if (camera->tile_count_x <= 0 || camera->tile_count_y <= 0 || tile_count_long >= 0x7fffffff) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_build_tile_lists: invalid tile grid");
     }
+    if (camera->tile_count_x > 0xffff || camera->tile_count_y > 0xffff) {
+        return fail(GSN_ERROR_CAPACITY, "gsn_build_tile_lists: more than 65535 tiles along an axis");
+    }
     int tile_count = static_cast<int>(tile_count_long);
     hipStream_t stream = context->stream;
     if (record_count == 0 || count == 0) {
@@ -165,24 +312,32 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
         GSN_TRY(check_launch("depth_keys_kernel"));
         GSN_TRY(radix_sort_pairs(context, depth_keys, depth_ids, count, 32, nullptr, sorted_ids));
     }
+    int tile_bits = 0;
+    for (unsigned t = static_cast<unsigned>(tile_count - 1); t; t >>= 1) {
+        ++tile_bits;
+    }
+    const int blocks = static_cast<int>((static_cast<int64_t>(record_count) + kRadixTile - 1) / kRadixTile);
+    uint32_t* sorted_tiles = static_cast<uint32_t*>(scratch(context, kSlotSortKeysB, r * 4));
+    int* block_owners = static_cast<int*>(scratch(context, kSlotEmitOwners, (static_cast<size_t>(blocks) + 1) * 4));
+    int* histogram = radix_histogram_buffer(context, record_count);
+    if (!sorted_tiles || !block_owners || !histogram) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
     {
         StageScope scope(context, kStageEmit);
         depth_order_counts_kernel<<<grid_for(count + 1, 256), 256, 0, stream>>>(sorted_ids, record_offsets,
             depth_counts, count);
         GSN_TRY(check_launch("depth_order_counts_kernel"));
         GSN_TRY(exclusive_scan_int(context, depth_counts, depth_counts, static_cast<int64_t>(count) + 1));
-        emit_records_kernel<<<grid_for(static_cast<int64_t>(count) * kEmitLanes, 256), 256, 0, stream>>>(sorted_ids,
-            depth_counts, reinterpret_cast<const int4*>(tile_rects), record_tiles, record_ids, count,
-            camera->tile_count_x, record_count);
+        emit_owners_kernel<<<grid_for(count, 256), 256, 0, stream>>>(depth_counts, count, record_count, blocks,
+            block_owners);
+        GSN_TRY(check_launch("emit_owners_kernel"));
+        // With a single tile (tile_bits == 0) the histogram is not used; one digit bit keeps the
+        // kernel's arguments in range.
+        emit_records_kernel<<<blocks, kEmitThreads, 0, stream>>>(sorted_ids, depth_counts, block_owners,
+            reinterpret_cast<const int4*>(tile_rects), record_tiles, record_ids, count, camera->tile_count_x,
+            record_count, blocks, radix_first_digit_bits(tile_bits), histogram);
         GSN_TRY(check_launch("emit_records_kernel"));
-    }
-    int tile_bits = 0;
-    for (unsigned t = static_cast<unsigned>(tile_count - 1); t; t >>= 1) {
-        ++tile_bits;
-    }
-    uint32_t* sorted_tiles = static_cast<uint32_t*>(scratch(context, kSlotSortKeysB, r * 4));
-    if (!sorted_tiles) {
-        return GSN_ERROR_OUT_OF_MEMORY;
     }
     {
         StageScope scope(context, kStageTileSort);
@@ -192,7 +347,7 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
         }
         else {
             GSN_TRY(radix_sort_pairs(context, record_tiles, record_ids, record_count, tile_bits, sorted_tiles,
-                reinterpret_cast<uint32_t*>(tile_gaussian_ids)));
+                reinterpret_cast<uint32_t*>(tile_gaussian_ids), true));
         }
     }
     // The ranges kernel writes every entry of tile_offsets[0..T] when R > 0.

@@ -122,6 +122,33 @@ def test_tile_lists_large_bit_exact(gsn, context, oracle, scenes):
     np.testing.assert_array_equal(actual["sorted_keys"], lists["sorted_keys"])
 
 
+def test_tile_lists_extreme_footprints_bit_exact(gsn, context, oracle, scenes):
+    """Records are emitted in 4096-record blocks: blocks owned by thousands of one- and two-tile
+    Gaussians, and full-screen Gaussians whose records span two or three blocks."""
+    tiny = scenes.blob_scene(40_000, seed=5, sh_degree=0)
+    tiny[3:6] = np.float32(-7.5)
+    huge = scenes.blob_scene(6, seed=6, sh_degree=0)
+    huge[0:3] = np.array([[0.0, 0.3, -0.4, 0.5, 0.2, -0.1], [0.0, -0.2, 0.1, 0.3, -0.3, 0.2],
+                          [3.0, 3.2, 2.8, 3.5, 2.5, 3.1]], np.float32)
+    huge[3:6] = np.float32(-0.3)
+    params = np.concatenate([tiny[:, :20_000], huge, tiny[:, 20_000:]], axis=1)
+    R, t = scenes.look_at(np.array([0.3, -0.8, 4.5]))
+    f = scenes.fov_focal(1920, 60.0)
+    cam = gsn.Camera.make(1920, 1080, f, f, 959.5, 539.5, R, t)
+    run = GpuRun(gsn, context, params, cam, sh_degree=0, keep_keys=True)
+    expected = oracle.project_f32(params, cam, 0)
+    _compare_projection(run, expected)
+    lists = oracle.tile_lists(cam, expected)
+    actual = run.lists()
+    counts = np.diff(lists["record_offsets"])
+    assert counts.max() > 4096, "some Gaussian spans two or more blocks"
+    assert (counts == 1).sum() > 2 * 4096, "blocks owned by one-record Gaussians"
+    assert actual["record_count"] == lists["record_count"]
+    np.testing.assert_array_equal(actual["tile_offsets"], lists["tile_offsets"])
+    np.testing.assert_array_equal(actual["tile_gaussian_ids"], lists["tile_gaussian_ids"])
+    np.testing.assert_array_equal(actual["sorted_keys"], lists["sorted_keys"])
+
+
 @pytest.mark.parametrize("sh_degree", [0, 3])
 def test_forward_matches_fp32_oracle(gsn, context, oracle, scenes, sh_degree):
     params = scenes.config0_params(seed=11, count=10_000, sh_degree=3)
