@@ -211,24 +211,46 @@ __global__ __launch_bounds__(kEmitThreads) void emit_records_kernel(const uint32
     }
 }
 
+// Four records per thread (one 16-byte load when the run is whole): a tile's range starts at
+// the first record whose tile differs from its predecessor's.
 __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* sorted_tiles, int64_t* offsets, int records,
     int tile_count) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= records) {
+    const int64_t first = 4 * (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+    if (first >= records) {
         return;
     }
-    // Tile ids come from rectangles clamped to the grid; min() keeps writes inside offsets[0..T]
-    // even for rectangles produced with a different camera.
-    int tile = min(static_cast<int>(sorted_tiles[r]), tile_count);
-    int previous = r == 0 ? -1 : min(static_cast<int>(sorted_tiles[r - 1]), tile_count);
-    if (tile != previous) {
+    int tiles[4];
+    if (first + 4 <= records) {
+        const uint4 quad = *reinterpret_cast<const uint4*>(sorted_tiles + first);
+        tiles[0] = static_cast<int>(quad.x);
+        tiles[1] = static_cast<int>(quad.y);
+        tiles[2] = static_cast<int>(quad.z);
+        tiles[3] = static_cast<int>(quad.w);
+    }
+    else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            tiles[i] = first + i < records ? static_cast<int>(sorted_tiles[first + i]) : tile_count;
+        }
+    }
+    // Tile ids come from rectangles clamped to the grid; the clamp keeps writes inside
+    // offsets[0..T] even for rectangles produced with a different camera.
+    int previous = first == 0 ? -1 : min(max(static_cast<int>(sorted_tiles[first - 1]), 0), tile_count);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t r = first + i;
+        if (r >= records) {
+            break;
+        }
+        const int tile = min(max(tiles[i], 0), tile_count);
         for (int t = previous + 1; t <= tile; ++t) {
             offsets[t] = r;
         }
-    }
-    if (r == records - 1) {
-        for (int t = tile + 1; t <= tile_count; ++t) {
-            offsets[t] = records;
+        previous = tile;
+        if (r == records - 1) {
+            for (int t = tile + 1; t <= tile_count; ++t) {
+                offsets[t] = records;
+            }
         }
     }
 }
@@ -352,8 +374,8 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
     }
     // The ranges kernel writes every entry of tile_offsets[0..T] when R > 0.
     StageScope scope(context, kStageRanges);
-    tile_ranges_kernel<<<grid_for(record_count, 256), 256, 0, stream>>>(sorted_tiles, tile_offsets, record_count,
-        tile_count);
+    tile_ranges_kernel<<<grid_for((static_cast<int64_t>(record_count) + 3) / 4, 256), 256, 0, stream>>>(sorted_tiles,
+        tile_offsets, record_count, tile_count);
     GSN_TRY(check_launch("tile_ranges_kernel"));
     if (sorted_keys) {
         sorted_keys_kernel<<<grid_for(record_count, 256), 256, 0, stream>>>(sorted_tiles, tile_gaussian_ids, depths,
