@@ -36,6 +36,8 @@ enum ScratchSlot {
     kSlotRecordFlags,
     kSlotTileOrder,
     kSlotEmitOwners,
+    kSlotDepthSortedKeys,
+    kSlotDepthRects,
     kSlotCount
 };
 

@@ -25,8 +25,12 @@ __global__ __launch_bounds__(256) void depth_keys_kernel(const float* depths, co
     ids[i] = static_cast<uint32_t>(i);
 }
 
-__global__ __launch_bounds__(256) void depth_order_counts_kernel(const uint32_t* sorted_ids, const int* record_offsets,
-    int* counts, int count) {
+// Record counts and tile rectangles in depth order: the one gather by Gaussian id of the tile
+// lists. A visible Gaussian's record count is its rectangle's area; invisible Gaussians (sorted
+// last, with the key 0xffffffff that no finite depth has) get none, and their rectangles, which
+// the projection leaves unwritten, are not read.
+__global__ __launch_bounds__(256) void depth_order_counts_kernel(const uint32_t* sorted_ids, const uint32_t* sorted_keys,
+    const int4* tile_rects, int* counts, int4* ordered_rects, int count) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > count) {
         return;
@@ -35,8 +39,12 @@ __global__ __launch_bounds__(256) void depth_order_counts_kernel(const uint32_t*
         counts[k] = 0;
         return;
     }
-    int id = static_cast<int>(sorted_ids[k]);
-    counts[k] = record_offsets[id + 1] - record_offsets[id];
+    int4 rect = make_int4(0, 0, 0, 0);
+    if (sorted_keys[k] != 0xffffffffu) {
+        rect = tile_rects[min(sorted_ids[k], static_cast<uint32_t>(count - 1))];
+    }
+    ordered_rects[k] = rect;
+    counts[k] = max(rect.z - rect.x, 0) * max(rect.w - rect.y, 0);
 }
 
 // Records are emitted in blocks of kRadixTile consecutive positions, one workgroup per block, so
@@ -75,7 +83,7 @@ __global__ __launch_bounds__(256) void emit_owners_kernel(const int* depth_offse
 // tile ids, which is the first pass's histogram column of the radix sort that follows (the
 // block is exactly one radix tile).
 __global__ __launch_bounds__(kEmitThreads) void emit_records_kernel(const uint32_t* __restrict__ sorted_ids,
-    const int* __restrict__ depth_offsets, const int* __restrict__ owners, const int4* __restrict__ tile_rects,
+    const int* __restrict__ depth_offsets, const int* __restrict__ owners, const int4* __restrict__ ordered_rects,
     uint32_t* __restrict__ record_tiles, uint32_t* __restrict__ record_ids, int count, int tile_count_x,
     int record_count, int blocks, int digit_bits, int* __restrict__ histogram) {
     __shared__ int s_owner[kRadixTile];
@@ -105,7 +113,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_records_kernel(const uint32
             int k = first + o;
             int start = depth_offsets[k];
             uint32_t id = min(sorted_ids[k], static_cast<uint32_t>(count - 1));
-            int4 rect = tile_rects[id];
+            int4 rect = ordered_rects[k];
             s_start[o] = start;
             s_id[o] = id;
             s_corner[o] = static_cast<uint32_t>(rect.x & 0xffff) | (static_cast<uint32_t>(rect.y) << 16);
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_records_kernel(const uint32
                 }
             }
             uint32_t id = min(sorted_ids[low], static_cast<uint32_t>(count - 1));
-            int4 rect = tile_rects[id];
+            int4 rect = ordered_rects[low];
             int width = max(rect.z - rect.x, 1);
             int local = r - depth_offsets[low];
             int row = local / width;
@@ -321,10 +329,13 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
     uint32_t* depth_keys = static_cast<uint32_t*>(scratch(context, kSlotDepthKeys, n * 4));
     uint32_t* depth_ids = static_cast<uint32_t*>(scratch(context, kSlotDepthIds, n * 4));
     uint32_t* sorted_ids = static_cast<uint32_t*>(scratch(context, kSlotDepthSortedIds, n * 4));
+    uint32_t* depth_sorted_keys = static_cast<uint32_t*>(scratch(context, kSlotDepthSortedKeys, n * 4));
+    int4* ordered_rects = static_cast<int4*>(scratch(context, kSlotDepthRects, n * sizeof(int4)));
     int* depth_counts = static_cast<int*>(scratch(context, kSlotDepthCounts, (n + 1) * 4));
     uint32_t* record_tiles = static_cast<uint32_t*>(scratch(context, kSlotRecordTiles, r * 4));
     uint32_t* record_ids = static_cast<uint32_t*>(scratch(context, kSlotRecordIds, r * 4));
-    if (!depth_keys || !depth_ids || !sorted_ids || !depth_counts || !record_tiles || !record_ids) {
+    if (!depth_keys || !depth_ids || !sorted_ids || !depth_sorted_keys || !ordered_rects || !depth_counts || !record_tiles ||
+        !record_ids) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
     {
@@ -332,7 +343,7 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
         depth_keys_kernel<<<grid_for(count, 256), 256, 0, stream>>>(depths, record_offsets, depth_keys, depth_ids,
             count);
         GSN_TRY(check_launch("depth_keys_kernel"));
-        GSN_TRY(radix_sort_pairs(context, depth_keys, depth_ids, count, 32, nullptr, sorted_ids));
+        GSN_TRY(radix_sort_pairs(context, depth_keys, depth_ids, count, 32, depth_sorted_keys, sorted_ids));
     }
     int tile_bits = 0;
     for (unsigned t = static_cast<unsigned>(tile_count - 1); t; t >>= 1) {
@@ -347,8 +358,8 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
     }
     {
         StageScope scope(context, kStageEmit);
-        depth_order_counts_kernel<<<grid_for(count + 1, 256), 256, 0, stream>>>(sorted_ids, record_offsets,
-            depth_counts, count);
+        depth_order_counts_kernel<<<grid_for(count + 1, 256), 256, 0, stream>>>(sorted_ids, depth_sorted_keys,
+            reinterpret_cast<const int4*>(tile_rects), depth_counts, ordered_rects, count);
         GSN_TRY(check_launch("depth_order_counts_kernel"));
         GSN_TRY(exclusive_scan_int(context, depth_counts, depth_counts, static_cast<int64_t>(count) + 1));
         emit_owners_kernel<<<grid_for(count, 256), 256, 0, stream>>>(depth_counts, count, record_count, blocks,
@@ -357,7 +368,7 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
         // With a single tile (tile_bits == 0) the histogram is not used; one digit bit keeps the
         // kernel's arguments in range.
         emit_records_kernel<<<blocks, kEmitThreads, 0, stream>>>(sorted_ids, depth_counts, block_owners,
-            reinterpret_cast<const int4*>(tile_rects), record_tiles, record_ids, count, camera->tile_count_x,
+            ordered_rects, record_tiles, record_ids, count, camera->tile_count_x,
             record_count, blocks, radix_first_digit_bits(tile_bits), histogram);
         GSN_TRY(check_launch("emit_records_kernel"));
     }
