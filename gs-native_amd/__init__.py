@@ -136,6 +136,7 @@ _PROTOTYPES = {
     "gsn_selftest_wave_sum12": [_P, _P, _P],
     "gsn_selftest_band_mask": [_P, _P, _P, _I, _P],
     "gsn_context_set_timing": [_P, _I],
+    "gsn_context_set_timing_mask": [_P, ctypes.c_uint],
     "gsn_context_stage_times": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), _I],
 }
 
@@ -268,8 +269,15 @@ class Context:
     def scratch_bytes(self) -> int:
         return int(lib.gsn_context_scratch_bytes(self._handle))
 
-    def set_timing(self, enabled: bool) -> None:
-        call("gsn_context_set_timing", self._handle, int(enabled))
+    def set_timing(self, enabled: bool, stages=None) -> None:
+        """Times every stage (or only the named `stages`) until disabled."""
+        if stages is None:
+            call("gsn_context_set_timing", self._handle, int(enabled))
+            return
+        mask = 0
+        for name in stages:
+            mask |= 1 << STAGES.index(name)
+        call("gsn_context_set_timing_mask", self._handle, mask if enabled else 0)
 
     def stage_times(self) -> dict:
         """{stage: (total milliseconds, timed launches)} since timing was enabled or last read."""

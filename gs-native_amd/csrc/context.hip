@@ -90,6 +90,10 @@ int gsn_context_create(gsn_context** context) {
     if (status == GSN_OK) {
         status = gsn::check_hip(hipHostMalloc(&created->pinned, 256, hipHostMallocDefault), "hipHostMalloc");
     }
+    if (status == GSN_OK) {
+        status = gsn::check_hip(hipEventCreateWithFlags(&created->readback_event, hipEventDisableTiming),
+            "hipEventCreateWithFlags");
+    }
     if (status != GSN_OK) {
         gsn_context_destroy(created);
         return status;
@@ -112,6 +116,9 @@ void gsn_context_destroy(gsn_context* context) {
     }
     if (context->pinned) {
         (void)hipHostFree(context->pinned);
+    }
+    if (context->readback_event) {
+        (void)hipEventDestroy(context->readback_event);
     }
     for (int k = 0; k < 2 * context->event_capacity; ++k) {
         if (context->events[k]) {
@@ -288,7 +295,7 @@ int gsn_event_elapsed_ms(void* start, void* end, float* milliseconds) {
 namespace gsn {
 
 StageScope::StageScope(gsn_context* context, Stage stage) : context_(context), pair_(-1) {
-    if (!context || !context->timing) {
+    if (!context || !((context->timing_mask >> stage) & 1u)) {
         return;
     }
     if (context->event_used == context->event_capacity) {
@@ -336,7 +343,16 @@ extern "C" int gsn_context_set_timing(gsn_context* context, int enabled) {
     if (!context) {
         return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_context_set_timing: null context");
     }
-    context->timing = enabled != 0;
+    context->timing_mask = enabled != 0 ? (1u << kStageCount) - 1u : 0u;
+    context->event_used = 0;
+    return GSN_OK;
+}
+
+extern "C" int gsn_context_set_timing_mask(gsn_context* context, unsigned stage_mask) {
+    if (!context) {
+        return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_context_set_timing_mask: null context");
+    }
+    context->timing_mask = stage_mask & ((1u << kStageCount) - 1u);
     context->event_used = 0;
     return GSN_OK;
 }

@@ -72,8 +72,18 @@ struct gsn_context {
     uint8_t record_epoch = 0;
     void* record_flags_cleared = nullptr;
     size_t record_flags_cleared_bytes = 0;
-    // Stage timing: event pairs recorded around kernel launches while enabled.
-    bool timing = false;
+    // Depth order queued ahead of gsn_build_tile_lists: the buffers of the last projection, and
+    // those whose order gsn_scan_record_offsets has queued (consumed by the next tile-list
+    // build). readback_event marks the record-count copy.
+    const float* projected_depths = nullptr;
+    const int* projected_offsets = nullptr;
+    int projected_count = -1;
+    const float* ordered_depths = nullptr;
+    const int* ordered_offsets = nullptr;
+    int ordered_count = -1;
+    hipEvent_t readback_event = nullptr;
+    // Stage timing: event pairs recorded around the launches of the stages in timing_mask.
+    unsigned timing_mask = 0;
     hipEvent_t* events = nullptr;
     int* event_stage = nullptr;
     int event_capacity = 0;  // number of pairs

@@ -196,6 +196,13 @@ extern "C" int gsn_project_gaussians(gsn_context* context, const float* params, 
     if (count > 0 && (!params || !means || !conic_opacities || !colors || !depths || !tile_rects)) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_project_gaussians: null buffer");
     }
+    // New projection outputs: any depth order queued for earlier ones is stale.
+    context->ordered_depths = nullptr;
+    context->ordered_offsets = nullptr;
+    context->ordered_count = -1;
+    context->projected_depths = count > 0 ? depths : nullptr;
+    context->projected_offsets = record_offsets;
+    context->projected_count = count;
     if (count == 0) {
         return check_hip(hipMemsetAsync(record_offsets, 0, sizeof(int), context->stream), "hipMemsetAsync");
     }

@@ -280,6 +280,25 @@ __global__ void zero_offsets_kernel(int64_t* offsets, int entries) {
 }
 
 } // namespace
+
+// Stable order of the N Gaussians by depth bits (visible ones first) into the depth-order slots:
+// sorted ids and sorted keys.
+int enqueue_depth_order(gsn_context* context, const float* depths, const int* record_offsets, int count) {
+    size_t n = static_cast<size_t>(count);
+    uint32_t* depth_keys = static_cast<uint32_t*>(scratch(context, kSlotDepthKeys, n * 4));
+    uint32_t* depth_ids = static_cast<uint32_t*>(scratch(context, kSlotDepthIds, n * 4));
+    uint32_t* sorted_ids = static_cast<uint32_t*>(scratch(context, kSlotDepthSortedIds, n * 4));
+    uint32_t* sorted_keys = static_cast<uint32_t*>(scratch(context, kSlotDepthSortedKeys, n * 4));
+    if (!depth_keys || !depth_ids || !sorted_ids || !sorted_keys) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    StageScope scope(context, kStageDepthSort);
+    depth_keys_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(depths, record_offsets, depth_keys, depth_ids,
+        count);
+    GSN_TRY(check_launch("depth_keys_kernel"));
+    return radix_sort_pairs(context, depth_keys, depth_ids, count, 32, sorted_keys, sorted_ids);
+}
+
 } // namespace gsn
 
 extern "C" int gsn_scan_record_offsets(gsn_context* context, int* record_offsets, int count, int* record_count) {
@@ -293,7 +312,22 @@ extern "C" int gsn_scan_record_offsets(gsn_context* context, int* record_offsets
     }
     int* pinned = static_cast<int*>(context->pinned);
     GSN_HIP(hipMemcpyAsync(pinned, record_offsets + count, sizeof(int), hipMemcpyDeviceToHost, context->stream));
-    GSN_HIP(hipStreamSynchronize(context->stream));
+    GSN_HIP(hipEventRecord(context->readback_event, context->stream));
+    // The depth order of the last projection does not need R: queue it behind the copy, so the
+    // GPU sorts while the caller reads R and queues the kernels that do.
+    const bool prepare = count > 0 && context->projected_depths && context->projected_offsets == record_offsets &&
+                         context->projected_count == count;
+    const float* depths = context->projected_depths;
+    context->projected_depths = nullptr;
+    context->projected_offsets = nullptr;
+    context->projected_count = -1;
+    if (prepare) {
+        GSN_TRY(enqueue_depth_order(context, depths, record_offsets, count));
+        context->ordered_depths = depths;
+        context->ordered_offsets = record_offsets;
+        context->ordered_count = count;
+    }
+    GSN_HIP(hipEventSynchronize(context->readback_event));
     *record_count = pinned[0];
     if (*record_count < 0) {
         return fail(GSN_ERROR_CAPACITY, "gsn_scan_record_offsets: record count overflowed int32");
@@ -326,24 +360,22 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
     }
     size_t n = static_cast<size_t>(count);
     size_t r = static_cast<size_t>(record_count);
-    uint32_t* depth_keys = static_cast<uint32_t*>(scratch(context, kSlotDepthKeys, n * 4));
-    uint32_t* depth_ids = static_cast<uint32_t*>(scratch(context, kSlotDepthIds, n * 4));
+    const bool ordered = context->ordered_depths == depths && context->ordered_offsets == record_offsets &&
+                         context->ordered_count == count;
+    context->ordered_depths = nullptr;
+    context->ordered_offsets = nullptr;
+    context->ordered_count = -1;
+    if (!ordered) {
+        GSN_TRY(enqueue_depth_order(context, depths, record_offsets, count));
+    }
     uint32_t* sorted_ids = static_cast<uint32_t*>(scratch(context, kSlotDepthSortedIds, n * 4));
     uint32_t* depth_sorted_keys = static_cast<uint32_t*>(scratch(context, kSlotDepthSortedKeys, n * 4));
     int4* ordered_rects = static_cast<int4*>(scratch(context, kSlotDepthRects, n * sizeof(int4)));
     int* depth_counts = static_cast<int*>(scratch(context, kSlotDepthCounts, (n + 1) * 4));
     uint32_t* record_tiles = static_cast<uint32_t*>(scratch(context, kSlotRecordTiles, r * 4));
     uint32_t* record_ids = static_cast<uint32_t*>(scratch(context, kSlotRecordIds, r * 4));
-    if (!depth_keys || !depth_ids || !sorted_ids || !depth_sorted_keys || !ordered_rects || !depth_counts || !record_tiles ||
-        !record_ids) {
+    if (!sorted_ids || !depth_sorted_keys || !ordered_rects || !depth_counts || !record_tiles || !record_ids) {
         return GSN_ERROR_OUT_OF_MEMORY;
-    }
-    {
-        StageScope scope(context, kStageDepthSort);
-        depth_keys_kernel<<<grid_for(count, 256), 256, 0, stream>>>(depths, record_offsets, depth_keys, depth_ids,
-            count);
-        GSN_TRY(check_launch("depth_keys_kernel"));
-        GSN_TRY(radix_sort_pairs(context, depth_keys, depth_ids, count, 32, depth_sorted_keys, sorted_ids));
     }
     int tile_bits = 0;
     for (unsigned t = static_cast<unsigned>(tile_count - 1); t; t >>= 1) {

@@ -88,6 +88,10 @@ int gsn_device_memory_info(size_t* free_bytes, size_t* total_bytes);
  *   10 Adam, 11 density control, 12 KNN. */
 #define GSN_STAGE_COUNT 13
 int gsn_context_set_timing(gsn_context* context, int enabled);
+/* Times only the stages whose bit (1 << stage) is set in stage_mask (0 disables timing). Each
+ * event record costs the stream a short idle gap, so a throughput measurement times only the
+ * stages it needs. No counterpart in the reference. */
+int gsn_context_set_timing_mask(gsn_context* context, unsigned stage_mask);
 int gsn_context_stage_times(gsn_context* context, double* milliseconds, int* launches, int stages);
 
 /* Diagnostic: the wave64 reduction used by the raster backward, on one wave.
@@ -119,7 +123,12 @@ int gsn_project_gaussians(gsn_context* context, const float* params, int count,
     float* colors, float* depths, int* tile_rects, int* record_offsets);
 
 /* In-place exclusive scan of record_offsets[N+1]; the total R is copied to *record_count
- * (synchronises the stream). Replaces the DeviceScan + readback at renderer.cu:465-487. */
+ * (waits for that copy). Replaces the DeviceScan + readback at renderer.cu:465-487.
+ * When record_offsets and count are those of the last gsn_project_gaussians call on this
+ * context, the call also queues the depth ordering of the visible Gaussians (using that call's
+ * depths) behind the copy, so the GPU sorts while the caller reads R; gsn_build_tile_lists with
+ * the same depths, record_offsets and count then reuses it. The depths must not change in
+ * between. */
 int gsn_scan_record_offsets(gsn_context* context, int* record_offsets, int count,
     int* record_count);
 
@@ -127,7 +136,8 @@ int gsn_scan_record_offsets(gsn_context* context, int* record_offsets, int count
  * Replaces WriteTileRecordsKernel, DeviceRadixSort::SortPairs and IdentifyTileOffsetsKernel
  * (renderer.cu:223-271, 534-569). Output order equals a stable sort of the reference keys
  * (tile << 32 | float_bits(depth)) over records enumerated in Gaussian-id order.
- * sorted_keys[R] is optional (NULL to skip). */
+ * sorted_keys[R] is optional (NULL to skip). Reuses the depth order queued by
+ * gsn_scan_record_offsets for the same depths / record_offsets / count, else computes it. */
 int gsn_build_tile_lists(gsn_context* context, const gsn_camera* camera, int count,
     const int* tile_rects, const float* depths, const int* record_offsets, int record_count,
     int* tile_gaussian_ids, int64_t* tile_offsets, uint64_t* sorted_keys);
