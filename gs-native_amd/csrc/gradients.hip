@@ -282,7 +282,7 @@ constexpr int kAdamGroup = 16;
 constexpr int kGradientBlock = 256;
 constexpr int kChunks = 2;  // 64-record chunks per wave pass
 
-template<int Degree, bool Fused>
+template<int Degree, bool Fused, bool Streaming>
 __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const float* __restrict__ partials,
     const int* __restrict__ record_offsets, const uint8_t* __restrict__ record_flags, uint8_t epoch, int count,
     int record_count, Camera cam, float* params, float* gradients, float* screen, float* statistics,
@@ -440,10 +440,12 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     };
     if (Fused) {
         // Adam over the 58 rows in groups: all loads of a group are issued before its stores, so
-        // each wave keeps 3 * kAdamGroup loads in flight instead of one row's. The rows are
-        // streamed once per step with non-temporal loads and stores, which keep them from
-        // displacing cached data (measured: 1.28 -> 1.15 ms at 3M Gaussians; non-temporal
-        // stores alone changed nothing).
+        // each wave keeps 3 * kAdamGroup loads in flight instead of one row's. With line-aligned
+        // rows (Streaming) they are streamed with non-temporal loads and stores, which keep them
+        // from displacing cached data (measured 1.28 -> 1.15 ms at 3M Gaussians). With rows off
+        // the line grid (a count that is not a multiple of 32, as after density control) each
+        // wave's row segment straddles three lines and non-temporal loads fetch the shared end
+        // lines twice (1.84 ms); ordinary accesses then take 1.28 ms.
         float* __restrict__ p = params;
         float* __restrict__ m = first;
         float* __restrict__ v = second;
@@ -454,9 +456,9 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
             for (int k = 0; k < kAdamGroup; ++k) {
                 if (base + k < kParamCount) {
                     int64_t index = (base + k) * n + id;
-                    pv[k] = __builtin_nontemporal_load(p + index);
-                    mv[k] = __builtin_nontemporal_load(m + index);
-                    vv[k] = __builtin_nontemporal_load(v + index);
+                    pv[k] = Streaming ? __builtin_nontemporal_load(p + index) : p[index];
+                    mv[k] = Streaming ? __builtin_nontemporal_load(m + index) : m[index];
+                    vv[k] = Streaming ? __builtin_nontemporal_load(v + index) : v[index];
                 }
             }
 #pragma unroll
@@ -466,9 +468,24 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
                     int64_t index = row * n + id;
                     adam_values(pv[k], mv[k], vv[k], row_gradient(row),
                         learning_rate_for_row(row, position_learning_rate), first_correction, second_correction);
-                    __builtin_nontemporal_store(pv[k], p + index);
-                    __builtin_nontemporal_store(mv[k], m + index);
-                    __builtin_nontemporal_store(vv[k], v + index);
+                    if (Streaming) {
+                        __builtin_nontemporal_store(pv[k], p + index);
+                    }
+                    else {
+                        p[index] = pv[k];
+                    }
+                    if (Streaming) {
+                        __builtin_nontemporal_store(mv[k], m + index);
+                    }
+                    else {
+                        m[index] = mv[k];
+                    }
+                    if (Streaming) {
+                        __builtin_nontemporal_store(vv[k], v + index);
+                    }
+                    else {
+                        v[index] = vv[k];
+                    }
                 }
             }
         }
@@ -528,27 +545,58 @@ int launch_gradients(gsn_context* context, const Camera& cam, int count, int rec
     float lr, float c1, float c2) {
     unsigned grid = grid_for(count, kGradientBlock);
     hipStream_t stream = context->stream;
+    // Non-temporal streaming of the Adam rows only when all of them start on a 128-byte line.
+    const bool streaming = Fused && rows_line_aligned(params, count) && rows_line_aligned(first, count) &&
+                           rows_line_aligned(second, count);
     StageScope scope(context, kStageGaussianGradients);
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
-        gaussian_gradients_kernel<0, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, record_flags,
-            epoch, count, record_count, cam, params,
-            gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
+        if (streaming) {
+            gaussian_gradients_kernel<0, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
+                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+                absolute_statistics, first, second, lr, c1, c2);
+        }
+        else {
+            gaussian_gradients_kernel<0, Fused, false><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
+                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+                absolute_statistics, first, second, lr, c1, c2);
+        }
         break;
     case 1:
-        gaussian_gradients_kernel<1, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, record_flags,
-            epoch, count, record_count, cam, params,
-            gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
+        if (streaming) {
+            gaussian_gradients_kernel<1, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
+                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+                absolute_statistics, first, second, lr, c1, c2);
+        }
+        else {
+            gaussian_gradients_kernel<1, Fused, false><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
+                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+                absolute_statistics, first, second, lr, c1, c2);
+        }
         break;
     case 2:
-        gaussian_gradients_kernel<2, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, record_flags,
-            epoch, count, record_count, cam, params,
-            gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
+        if (streaming) {
+            gaussian_gradients_kernel<2, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
+                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+                absolute_statistics, first, second, lr, c1, c2);
+        }
+        else {
+            gaussian_gradients_kernel<2, Fused, false><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
+                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+                absolute_statistics, first, second, lr, c1, c2);
+        }
         break;
     default:
-        gaussian_gradients_kernel<3, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets, record_flags,
-            epoch, count, record_count, cam, params,
-            gradients, screen, statistics, absolute_statistics, first, second, lr, c1, c2);
+        if (streaming) {
+            gaussian_gradients_kernel<3, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
+                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+                absolute_statistics, first, second, lr, c1, c2);
+        }
+        else {
+            gaussian_gradients_kernel<3, Fused, false><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
+                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+                absolute_statistics, first, second, lr, c1, c2);
+        }
         break;
     }
     return check_launch("gaussian_gradients_kernel");

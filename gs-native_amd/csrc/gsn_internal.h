@@ -156,6 +156,13 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
     const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
     const uint8_t** flags, uint8_t* epoch, const StoredState* stored = nullptr);
 
+// True when every row of a [rows][count] float array starts on a 128-byte line: a wave's
+// 256-byte segment of a row then covers two whole lines (the condition under which the streaming
+// kernels use non-temporal accesses).
+inline bool rows_line_aligned(const void* base, int64_t count) {
+    return count % 32 == 0 && reinterpret_cast<uintptr_t>(base) % 128 == 0;
+}
+
 inline unsigned grid_for(int64_t n, int block) {
     return static_cast<unsigned>((n + block - 1) / block);
 }

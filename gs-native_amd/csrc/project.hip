@@ -12,15 +12,19 @@ namespace {
 
 constexpr float kNearDegenerate = 1e-3f;
 
-// Parameter rows are read once per projection: non-temporal loads keep them from displacing data
-// that later kernels of the step read (measured 0.22 -> 0.17 ms at 3M Gaussians, SH degree 3).
+// Parameter rows are read once per projection: when every row starts on a 128-byte line (count a
+// multiple of 32, aligned buffer), non-temporal loads keep them from displacing data that later
+// kernels of the step read (measured 0.22 -> 0.17 ms at 3M Gaussians, SH degree 3). Otherwise a
+// wave's 256-byte row segment straddles three lines, and ordinary loads, which leave the shared
+// end lines in L2 for the neighbouring wave, are faster.
+template<bool Streaming>
 __device__ __forceinline__ float stream_load(const float* address) {
-    return __builtin_nontemporal_load(address);
+    return Streaming ? __builtin_nontemporal_load(address) : *address;
 }
 
 // Degree is a template parameter so that the SH loops unroll and all coefficient loads of a
 // Gaussian are in flight together (a runtime-bounded loop issued them one dependent step at a time).
-template<int Degree>
+template<int Degree, bool Streaming>
 __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ params, Camera camera, float2* means,
     float4* conic_opacities, float* colors, float* depths, int4* tile_rects, int* record_counts, int count) {
 #pragma clang fp contract(off)
@@ -34,9 +38,9 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     }
     record_counts[i] = 0;
     const int64_t n = count;
-    float px0 = stream_load(params + (kPositionIndex + 0) * n + i);
-    float py0 = stream_load(params + (kPositionIndex + 1) * n + i);
-    float pz0 = stream_load(params + (kPositionIndex + 2) * n + i);
+    float px0 = stream_load<Streaming>(params + (kPositionIndex + 0) * n + i);
+    float py0 = stream_load<Streaming>(params + (kPositionIndex + 1) * n + i);
+    float pz0 = stream_load<Streaming>(params + (kPositionIndex + 2) * n + i);
     const float* Rc = camera.R_c_w;
     const float* tc = camera.t_c_w;
     float px = Rc[0] * px0 + Rc[1] * py0 + Rc[2] * pz0 + tc[0];
@@ -47,11 +51,11 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
         return;
     }
     float Rg[9];
-    rotation_from_axis_angle(stream_load(params + (kRotationIndex + 0) * n + i),
-        stream_load(params + (kRotationIndex + 1) * n + i), stream_load(params + (kRotationIndex + 2) * n + i), Rg);
-    float scale[3] = {det_expf(stream_load(params + (kLogScaleIndex + 0) * n + i)),
-        det_expf(stream_load(params + (kLogScaleIndex + 1) * n + i)),
-        det_expf(stream_load(params + (kLogScaleIndex + 2) * n + i))};
+    rotation_from_axis_angle(stream_load<Streaming>(params + (kRotationIndex + 0) * n + i),
+        stream_load<Streaming>(params + (kRotationIndex + 1) * n + i), stream_load<Streaming>(params + (kRotationIndex + 2) * n + i), Rg);
+    float scale[3] = {det_expf(stream_load<Streaming>(params + (kLogScaleIndex + 0) * n + i)),
+        det_expf(stream_load<Streaming>(params + (kLogScaleIndex + 1) * n + i)),
+        det_expf(stream_load<Streaming>(params + (kLogScaleIndex + 2) * n + i))};
     float M[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -126,7 +130,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
         conic.y = -b * det_inv;
         conic.z = a * det_inv;
     }
-    conic.w = 1.0f / (1.0f + det_expf(-stream_load(params + kOpacityIndex * n + i)));
+    conic.w = 1.0f / (1.0f + det_expf(-stream_load<Streaming>(params + kOpacityIndex * n + i)));
     means[i] = make_float2(u, v);
     conic_opacities[i] = conic;
 
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     for (int ch = 0; ch < 3; ++ch) {
 #pragma unroll
         for (int j = 0; j < coeffs; ++j) {
-            coefficients[ch][j] = stream_load(params + (kShIndex + ch * kShCoeffCount + j) * n + i);
+            coefficients[ch][j] = stream_load<Streaming>(params + (kShIndex + ch * kShCoeffCount + j) * n + i);
         }
     }
 #pragma unroll
@@ -214,18 +218,47 @@ extern "C" int gsn_project_gaussians(gsn_context* context, const float* params, 
     float2* m = reinterpret_cast<float2*>(means);
     float4* q = reinterpret_cast<float4*>(conic_opacities);
     int4* r = reinterpret_cast<int4*>(tile_rects);
+    const bool streaming = rows_line_aligned(params, count);
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
-        project_kernel<0><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r, record_offsets, count);
+        if (streaming) {
+            project_kernel<0, true><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r,
+                record_offsets, count);
+        }
+        else {
+            project_kernel<0, false><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r,
+                record_offsets, count);
+        }
         break;
     case 1:
-        project_kernel<1><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r, record_offsets, count);
+        if (streaming) {
+            project_kernel<1, true><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r,
+                record_offsets, count);
+        }
+        else {
+            project_kernel<1, false><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r,
+                record_offsets, count);
+        }
         break;
     case 2:
-        project_kernel<2><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r, record_offsets, count);
+        if (streaming) {
+            project_kernel<2, true><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r,
+                record_offsets, count);
+        }
+        else {
+            project_kernel<2, false><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r,
+                record_offsets, count);
+        }
         break;
     default:
-        project_kernel<3><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r, record_offsets, count);
+        if (streaming) {
+            project_kernel<3, true><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r,
+                record_offsets, count);
+        }
+        else {
+            project_kernel<3, false><<<grid, 256, 0, context->stream>>>(params, cam, m, q, colors, depths, r,
+                record_offsets, count);
+        }
         break;
     }
     return check_launch("project_kernel");

@@ -149,6 +149,31 @@ def test_tile_lists_extreme_footprints_bit_exact(gsn, context, oracle, scenes):
     np.testing.assert_array_equal(actual["sorted_keys"], lists["sorted_keys"])
 
 
+def test_tile_lists_rebuilt_without_queued_depth_order(gsn, context, scenes):
+    """render() runs projection -> scan -> tile lists, and the scan queues the depth order that the
+    tile-list build reuses. A second build with the same buffers finds no queued order (it is
+    consumed) and sorts by depth itself; both builds must agree bit for bit."""
+    params = scenes.blob_scene(50_000, seed=9, sh_degree=0)
+    params[3:6] = np.float32(-3.5)
+    R, t = scenes.look_at(np.array([0.2, -0.5, 4.0]))
+    f = scenes.fov_focal(640, 60.0)
+    cam = gsn.Camera.make(640, 480, f, f, 319.5, 239.5, R, t)
+    run = GpuRun(gsn, context, params, cam, sh_degree=0, keep_keys=True)
+    first = run.lists()
+    p = run.projection
+    r = p.record_count
+    ids = gsn.DeviceBuffer((max(r, 1),), np.int32)
+    offsets = gsn.DeviceBuffer((cam.tile_count + 1,), np.int64)
+    keys = gsn.DeviceBuffer((max(r, 1),), np.uint64)
+    gsn.call("gsn_build_tile_lists", context.handle, ctypes.byref(cam), p.count, p.tile_rects.ptr, p.depths.ptr,
+             p.record_offsets.ptr, r, ids.ptr, offsets.ptr, keys.ptr)
+    context.synchronize()
+    assert r > 10_000
+    np.testing.assert_array_equal(offsets.numpy(), first["tile_offsets"])
+    np.testing.assert_array_equal(ids.numpy()[:r], first["tile_gaussian_ids"])
+    np.testing.assert_array_equal(keys.numpy()[:r], first["sorted_keys"])
+
+
 @pytest.mark.parametrize("sh_degree", [0, 3])
 def test_forward_matches_fp32_oracle(gsn, context, oracle, scenes, sh_degree):
     params = scenes.config0_params(seed=11, count=10_000, sh_degree=3)
