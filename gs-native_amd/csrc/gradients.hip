@@ -279,11 +279,12 @@ __device__ __forceinline__ void adam_entry(float* params, float* first, float* s
 
 constexpr int kAdamGroup = 16;
 
-constexpr int kGradientBlock = 256;
+constexpr int kGradientBlock = kGradientGroup;
 constexpr int kChunks = 2;  // 64-record chunks per wave pass
 
 template<int Degree, bool Fused, bool Streaming>
-__global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const float* __restrict__ partials,
+__global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const int* __restrict__ group_order,
+    const float* __restrict__ partials,
     const int* __restrict__ record_offsets, const uint8_t* __restrict__ record_flags, uint8_t epoch, int count,
     int record_count, Camera cam, float* params, float* gradients, float* screen, float* statistics,
     float* absolute_statistics, float* first, float* second, float position_learning_rate, float first_correction,
@@ -301,7 +302,9 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     __shared__ float4 s_records[kGradientBlock / 64][kChunks * 64 * 3];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int id = blockIdx.x * kGradientBlock + threadIdx.x;
+    // Workgroups run heaviest record run first (group_order); results do not depend on the order.
+    const int group = group_order ? group_order[blockIdx.x] : static_cast<int>(blockIdx.x);
+    const int id = group * kGradientBlock + threadIdx.x;
     const int64_t n = count;
     int begin = record_count, end = record_count;
     if (id < count) {
@@ -540,9 +543,9 @@ __global__ __launch_bounds__(256) void zero_rows_kernel(float* first, float* sec
 
 template<bool Fused>
 int launch_gradients(gsn_context* context, const Camera& cam, int count, int record_count, int sh_degree,
-    const float* partials, const int* record_offsets, const uint8_t* record_flags, uint8_t epoch, float* params,
-    float* gradients, float* screen, float* statistics, float* absolute_statistics, float* first, float* second,
-    float lr, float c1, float c2) {
+    const int* group_order, const float* partials, const int* record_offsets, const uint8_t* record_flags,
+    uint8_t epoch, float* params, float* gradients, float* screen, float* statistics, float* absolute_statistics,
+    float* first, float* second, float lr, float c1, float c2) {
     unsigned grid = grid_for(count, kGradientBlock);
     hipStream_t stream = context->stream;
     // Non-temporal streaming of the Adam rows only when all of them start on a 128-byte line.
@@ -552,49 +555,49 @@ int launch_gradients(gsn_context* context, const Camera& cam, int count, int rec
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
         if (streaming) {
-            gaussian_gradients_kernel<0, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
-                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+            gaussian_gradients_kernel<0, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
+                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
                 absolute_statistics, first, second, lr, c1, c2);
         }
         else {
-            gaussian_gradients_kernel<0, Fused, false><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
-                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+            gaussian_gradients_kernel<0, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
+                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
                 absolute_statistics, first, second, lr, c1, c2);
         }
         break;
     case 1:
         if (streaming) {
-            gaussian_gradients_kernel<1, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
-                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+            gaussian_gradients_kernel<1, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
+                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
                 absolute_statistics, first, second, lr, c1, c2);
         }
         else {
-            gaussian_gradients_kernel<1, Fused, false><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
-                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+            gaussian_gradients_kernel<1, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
+                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
                 absolute_statistics, first, second, lr, c1, c2);
         }
         break;
     case 2:
         if (streaming) {
-            gaussian_gradients_kernel<2, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
-                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+            gaussian_gradients_kernel<2, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
+                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
                 absolute_statistics, first, second, lr, c1, c2);
         }
         else {
-            gaussian_gradients_kernel<2, Fused, false><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
-                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+            gaussian_gradients_kernel<2, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
+                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
                 absolute_statistics, first, second, lr, c1, c2);
         }
         break;
     default:
         if (streaming) {
-            gaussian_gradients_kernel<3, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
-                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+            gaussian_gradients_kernel<3, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
+                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
                 absolute_statistics, first, second, lr, c1, c2);
         }
         else {
-            gaussian_gradients_kernel<3, Fused, false><<<grid, kGradientBlock, 0, stream>>>(partials, record_offsets,
-                record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
+            gaussian_gradients_kernel<3, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
+                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
                 absolute_statistics, first, second, lr, c1, c2);
         }
         break;
@@ -646,13 +649,15 @@ extern "C" int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* ca
     }
     const uint8_t* flags = nullptr;
     uint8_t epoch = 0;
+    const int* group_order = nullptr;
     if (record_count > 0) {
         GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
-            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count, &flags, &epoch));
+            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count, &flags, &epoch,
+            &group_order));
     }
-    return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets, flags,
-        epoch, const_cast<float*>(params), gradients, screen_gradients, statistics, absolute_statistics, nullptr,
-        nullptr, 0.0f, 0.0f, 0.0f);
+    return launch_gradients<false>(context, cam, count, record_count, sh_degree, group_order, partials,
+        record_offsets, flags, epoch, const_cast<float*>(params), gradients, screen_gradients, statistics,
+        absolute_statistics, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
 }
 
 extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, int count, int sh_degree,
@@ -680,13 +685,15 @@ extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camer
     }
     const uint8_t* flags = nullptr;
     uint8_t epoch = 0;
+    const int* group_order = nullptr;
     if (record_count > 0) {
         GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
-            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count, &flags, &epoch));
+            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count, &flags, &epoch,
+            &group_order));
     }
-    return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, flags,
-        epoch, params, nullptr, nullptr, statistics, absolute_statistics, first_moments, second_moments,
-        position_learning_rate, first_correction, second_correction);
+    return launch_gradients<true>(context, cam, count, record_count, sh_degree, group_order, partials,
+        record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, first_moments,
+        second_moments, position_learning_rate, first_correction, second_correction);
 }
 
 extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_camera* camera, int count,
@@ -715,15 +722,17 @@ extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_cam
     }
     const uint8_t* flags = nullptr;
     uint8_t epoch = 0;
+    const int* group_order = nullptr;
     if (record_count > 0) {
         StoredState stored{final_transmittance, last_contributor, {background[0], background[1], background[2]}};
         GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
             colors, tile_rects, record_offsets, nullptr, image_gradients, partials, record_count, &flags, &epoch,
+            &group_order,
             &stored));
     }
-    return launch_gradients<false>(context, cam, count, record_count, sh_degree, partials, record_offsets, flags, epoch,
-        const_cast<float*>(params), gradients, screen_gradients, statistics, absolute_statistics, nullptr, nullptr,
-        0.0f, 0.0f, 0.0f);
+    return launch_gradients<false>(context, cam, count, record_count, sh_degree, group_order, partials,
+        record_offsets, flags, epoch, const_cast<float*>(params), gradients, screen_gradients, statistics,
+        absolute_statistics, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
 }
 
 extern "C" int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gsn_camera* camera, int count,
@@ -752,15 +761,17 @@ extern "C" int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gs
     }
     const uint8_t* flags = nullptr;
     uint8_t epoch = 0;
+    const int* group_order = nullptr;
     if (record_count > 0) {
         StoredState stored{final_transmittance, last_contributor, {background[0], background[1], background[2]}};
         GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
             colors, tile_rects, record_offsets, nullptr, image_gradients, partials, record_count, &flags, &epoch,
+            &group_order,
             &stored));
     }
-    return launch_gradients<true>(context, cam, count, record_count, sh_degree, partials, record_offsets, flags,
-        epoch, params, nullptr, nullptr, statistics, absolute_statistics, first_moments, second_moments, position_learning_rate,
-        first_correction, second_correction);
+    return launch_gradients<true>(context, cam, count, record_count, sh_degree, group_order, partials,
+        record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, first_moments,
+        second_moments, position_learning_rate, first_correction, second_correction);
 }
 
 extern "C" int gsn_adam_update(gsn_context* context, int count, const float* gradients, float* params,

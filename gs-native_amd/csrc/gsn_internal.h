@@ -38,6 +38,7 @@ enum ScratchSlot {
     kSlotEmitOwners,
     kSlotDepthSortedKeys,
     kSlotDepthRects,
+    kSlotGaussianOrder,
     kSlotCount
 };
 
@@ -139,6 +140,9 @@ int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int
 // |dmean2D|(2), pad.
 constexpr int kPartialFloats = 12;
 
+// Gaussians per workgroup of the per-Gaussian gradient pass.
+constexpr int kGradientGroup = 256;
+
 // Per-pixel forward state consumed by the stored-state backward.
 struct StoredState {
     const float* final_transmittance;
@@ -154,7 +158,8 @@ struct StoredState {
 int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
     const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
     const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
-    const uint8_t** flags, uint8_t* epoch, const StoredState* stored = nullptr);
+    const uint8_t** flags, uint8_t* epoch, const int** gaussian_order,
+    const StoredState* stored = nullptr);
 
 // True when every row of a [rows][count] float array starts on a 128-byte line: a wave's
 // 256-byte segment of a row then covers two whole lines (the condition under which the streaming

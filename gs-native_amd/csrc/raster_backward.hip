@@ -60,24 +60,29 @@ __device__ __forceinline__ int64_t partial_slot(float4 c, int tile_x, int tile_y
 constexpr int kScheduleBins = 128;
 constexpr int kScheduleThreads = 1024;
 
-__device__ __forceinline__ int schedule_bin(const int64_t* tile_offsets, int tile) {
-    int64_t records = tile_offsets[tile + 1] - tile_offsets[tile];
+__device__ __forceinline__ int schedule_bin(int64_t records) {
     uint32_t v = static_cast<uint32_t>(min(max(records, int64_t{0}), int64_t{0x7ffffffe})) + 1u;
     int octave = 31 - __clz(v);
     int fraction = octave >= 2 ? static_cast<int>((v >> (octave - 2)) & 3u) : static_cast<int>((v << (2 - octave)) & 3u);
     return kScheduleBins - 1 - (octave * 4 + fraction);
 }
 
-__global__ __launch_bounds__(kScheduleThreads) void tile_schedule_kernel(const int64_t* __restrict__ tile_offsets,
-    int tile_count, int* __restrict__ order) {
-    __shared__ int histogram[kScheduleBins];
-    __shared__ int cursor[kScheduleBins];
+// Heavy-first order of `units` work units; unit u owns records [offsets[u * stride],
+// offsets[min((u + 1) * stride, count)]). Run by the whole (single) workgroup.
+template<typename Offset>
+__device__ void schedule_units(const Offset* __restrict__ offsets, int stride, int count, int units,
+    int* __restrict__ order, int* histogram, int* cursor) {
+    auto bin = [&](int u) {
+        int64_t begin = offsets[min(static_cast<int64_t>(u) * stride, static_cast<int64_t>(count))];
+        int64_t end = offsets[min(static_cast<int64_t>(u + 1) * stride, static_cast<int64_t>(count))];
+        return schedule_bin(end - begin);
+    };
     for (int b = threadIdx.x; b < kScheduleBins; b += kScheduleThreads) {
         histogram[b] = 0;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < tile_count; t += kScheduleThreads) {
-        atomicAdd(&histogram[schedule_bin(tile_offsets, t)], 1);
+    for (int u = threadIdx.x; u < units; u += kScheduleThreads) {
+        atomicAdd(&histogram[bin(u)], 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -88,9 +93,25 @@ __global__ __launch_bounds__(kScheduleThreads) void tile_schedule_kernel(const i
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < tile_count; t += kScheduleThreads) {
-        order[atomicAdd(&cursor[schedule_bin(tile_offsets, t)], 1)] = t;
+    for (int u = threadIdx.x; u < units; u += kScheduleThreads) {
+        order[atomicAdd(&cursor[bin(u)], 1)] = u;
     }
+    __syncthreads();
+}
+
+// Two schedules in one launch: the backward's tiles by record count, and the per-Gaussian pass's
+// workgroups (kGradientGroup consecutive Gaussians) by the length of their record run. Density
+// control appends its clones and split children, high-gradient Gaussians whose records are all
+// reached, at the end of the id range; in id order their workgroups, the longest of the pass,
+// were dispatched last and set its end (measured 1.3 -> 2.3 ms after one density step at 3M).
+__global__ __launch_bounds__(kScheduleThreads) void tile_schedule_kernel(const int64_t* __restrict__ tile_offsets,
+    int tile_count, int* __restrict__ order, const int* __restrict__ record_offsets, int gaussian_count,
+    int* __restrict__ gaussian_order) {
+    __shared__ int histogram[kScheduleBins];
+    __shared__ int cursor[kScheduleBins];
+    schedule_units(tile_offsets, 1, tile_count, tile_count, order, histogram, cursor);
+    schedule_units(record_offsets, kGradientGroup, gaussian_count,
+        (gaussian_count + kGradientGroup - 1) / kGradientGroup, gaussian_order, histogram, cursor);
 }
 
 // Plain stores: non-temporal stores of these scattered 16-byte pieces measured 2.3x the write
@@ -471,7 +492,7 @@ __global__ __launch_bounds__(256) void clear_flags_kernel(uint32_t* flags, int64
 int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
     const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
     const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
-    const uint8_t** flags, uint8_t* epoch, const StoredState* stored) {
+    const uint8_t** flags, uint8_t* epoch, const int** gaussian_order, const StoredState* stored) {
     if (cam.tile_count_x > 0xffff || cam.tile_count_y > 0x7fff) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "raster backward: tile grid exceeds 65535 x 32767");
     }
@@ -479,11 +500,13 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
     PackedSplat* splats = static_cast<PackedSplat*>(
         scratch(context, kSlotSplats, static_cast<size_t>(count > 0 ? count : 1) * sizeof(PackedSplat)));
     int* order = static_cast<int*>(scratch(context, kSlotTileOrder, static_cast<size_t>(tiles) * 4));
+    const int groups = (count + kGradientGroup - 1) / kGradientGroup;
+    int* groups_order = static_cast<int*>(scratch(context, kSlotGaussianOrder, static_cast<size_t>(groups > 0 ? groups : 1) * 4));
     // One flag byte per record, read 16 at a time by the per-Gaussian pass (scratch slots are
     // allocated in multiples of 256 bytes, so an aligned 16-byte read below the record count
     // stays inside the allocation).
     uint8_t* record_flags = static_cast<uint8_t*>(scratch(context, kSlotRecordFlags, static_cast<size_t>(record_count)));
-    if (!splats || !order || !record_flags) {
+    if (!splats || !order || !groups_order || !record_flags) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
     // The epoch runs 1..255. After a wrap, or in a newly allocated buffer, every byte is cleared
@@ -508,7 +531,9 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
         reinterpret_cast<const float4*>(conics), colors, reinterpret_cast<const int4*>(rects), record_offsets, count,
         splats);
     GSN_TRY(check_launch("pack_splats_kernel"));
-    tile_schedule_kernel<<<1, kScheduleThreads, 0, context->stream>>>(tile_offsets, static_cast<int>(tiles), order);
+    tile_schedule_kernel<<<1, kScheduleThreads, 0, context->stream>>>(tile_offsets, static_cast<int>(tiles), order,
+        record_offsets, count, groups_order);
+    *gaussian_order = groups_order;
     GSN_TRY(check_launch("tile_schedule_kernel"));
     if (stored) {
         raster_backward_stored_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets,
