@@ -21,7 +21,10 @@ struct Mapped {
 
 // Real SH basis function k (degree <= 3) at the unit direction (x, y, z); constants as the
 // reference's ComputeEquationSh (solver.cu:223-323). k is a compile-time constant after unrolling.
+// No FP contraction: otherwise the fused multiply-adds chosen for the polynomials depend on the
+// calling kernel, and the fused passes must give bit-identical SH gradients.
 __device__ __forceinline__ float sh_basis_value(int k, float x, float y, float z) {
+#pragma clang fp contract(off)
     switch (k) {
     case 0: return kShC0;
     case 1: return -kShC1 * y;
@@ -282,45 +285,53 @@ constexpr int kAdamGroup = 16;
 constexpr int kGradientBlock = kGradientGroup;
 constexpr int kChunks = 2;  // 64-record chunks per wave pass
 
-template<int Degree, bool Fused, bool Streaming>
-__global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const int* __restrict__ group_order,
-    const float* __restrict__ partials,
-    const int* __restrict__ record_offsets, const uint8_t* __restrict__ record_flags, uint8_t epoch, int count,
-    int record_count, Camera cam, float* params, float* gradients, float* screen, float* statistics,
-    float* absolute_statistics, float* first, float* second, float position_learning_rate, float first_correction,
-    float second_correction) {
-    // Each Gaussian owns the contiguous run of partial records of its tile rectangle, so a wave's 64
-    // Gaussians own one contiguous run. The backward wrote a partial only for the records its tile
-    // reached before every pixel was done, and stamped those records' flag bytes with this step's
-    // epoch (at 1080p with a converged scene about 4% of the records). The wave walks its run
-    // kChunks x 64 records at a time, one record per lane and chunk: the flag bytes are read
-    // together, the stamped partials are loaded (coalesced, neighbouring lanes hold neighbouring
-    // records) into the wave's LDS slots, and each lane then adds the stamped records of its own
-    // Gaussian from LDS in record order, so results are identical run to run. Work per wave
-    // follows the length of its run, not the largest rectangle in it, and each pass costs two
-    // dependent global accesses (flags, then partials) for kChunks x 64 records.
-    __shared__ float4 s_records[kGradientBlock / 64][kChunks * 64 * 3];
+// Projection statistics of one Gaussian (reference CollectProjectionStatisticsKernel,
+// solver.cu:468-490): pixel-space norms of the mean gradient and of its absolute counterpart, and
+// whether the Gaussian had records. No FP contraction, so that every kernel writes identical bits.
+__device__ __forceinline__ void write_statistics(const Camera& cam, const float (&in)[11], int id, int64_t n,
+    int begin, int end, float* statistics, float* absolute_statistics) {
+#pragma clang fp contract(off)
+    float x = in[0] * (0.5f * static_cast<float>(cam.width));
+    float y = in[1] * (0.5f * static_cast<float>(cam.height));
+    float ax = in[9] * (0.5f * static_cast<float>(cam.width));
+    float ay = in[10] * (0.5f * static_cast<float>(cam.height));
+    statistics[id] = sqrtf(x * x + y * y);
+    statistics[n + id] = end > begin ? 1.0f : 0.0f;
+    absolute_statistics[id] = sqrtf(ax * ax + ay * ay);
+    absolute_statistics[n + id] = in[10];
+}
+
+// Sums of the stamped partials of Gaussian `id` (Gaussians at or past `limit` own no records);
+// all 64 lanes of the wave call it together. slots: the wave's LDS staging area,
+// kChunks * 64 * 3 float4.
+// Each Gaussian owns the contiguous run of partial records of its tile rectangle, so a wave's 64
+// Gaussians own one contiguous run. The backward wrote a partial only for the records its tile
+// reached before every pixel was done, and stamped those records' flag bytes with this step's
+// epoch (at 1080p with a converged scene about 4% of the records). The wave walks its run
+// kChunks x 64 records at a time, one record per lane and chunk: the flag bytes are read
+// together, the stamped partials are loaded (coalesced, neighbouring lanes hold neighbouring
+// records) into the wave's LDS slots, and each lane then adds the stamped records of its own
+// Gaussian from LDS in record order, so results are identical run to run. Work per wave
+// follows the length of its run, not the largest rectangle in it, and each pass costs two
+// dependent global accesses (flags, then partials) for kChunks x 64 records.
+__device__ __forceinline__ void reduce_records(const float* __restrict__ partials,
+    const int* __restrict__ record_offsets, const uint8_t* __restrict__ record_flags, uint8_t epoch, int id, int limit,
+    int record_count, float4* slots, float (&in)[11], int& begin, int& end) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    // Workgroups run heaviest record run first (group_order); results do not depend on the order.
-    const int group = group_order ? group_order[blockIdx.x] : static_cast<int>(blockIdx.x);
-    const int id = group * kGradientBlock + threadIdx.x;
-    const int64_t n = count;
-    int begin = record_count, end = record_count;
-    if (id < count) {
+    begin = record_count;
+    end = record_count;
+    if (id < limit) {
         begin = min(record_offsets[id], record_count);
         end = min(record_offsets[id + 1], record_count);
     }
-    const int last_lane = min(63, max(0, count - 1 - (id - lane)));
+    const int last_lane = min(63, max(0, limit - 1 - (id - lane)));
     const int wave_begin = __shfl(begin, 0);
     const int wave_end = __shfl(end, last_lane);
-    float in[11];
 #pragma unroll
     for (int k = 0; k < 11; ++k) {
         in[k] = 0.0f;
     }
     const float4* records = reinterpret_cast<const float4*>(partials);
-    float4* slots = s_records[wave];
     // Flags of the first pass; each pass then loads the next pass's flags while its partials are
     // in flight, so a pass waits on memory about once. Flag reads past the run clamp to its last
     // record (they are never used).
@@ -397,19 +408,29 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
         }
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+template<int Degree, bool Fused, bool Streaming>
+__global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const int* __restrict__ group_order,
+    const float* __restrict__ partials,
+    const int* __restrict__ record_offsets, const uint8_t* __restrict__ record_flags, uint8_t epoch, int count,
+    int record_count, Camera cam, float* params, float* gradients, float* screen, float* statistics,
+    float* absolute_statistics, float* first, float* second, float position_learning_rate, float first_correction,
+    float second_correction) {
+    __shared__ float4 s_records[kGradientBlock / 64][kChunks * 64 * 3];
+    const int wave = threadIdx.x >> 6;
+    // Workgroups run heaviest record run first (group_order); results do not depend on the order.
+    const int group = group_order ? group_order[blockIdx.x] : static_cast<int>(blockIdx.x);
+    const int id = group * kGradientBlock + threadIdx.x;
+    const int64_t n = count;
+    float in[11];
+    int begin, end;
+    reduce_records(partials, record_offsets, record_flags, epoch, id, count, record_count, s_records[wave], in, begin,
+        end);
     if (id >= count) {
         return;
     }
-    {
-        float x = in[0] * (0.5f * static_cast<float>(cam.width));
-        float y = in[1] * (0.5f * static_cast<float>(cam.height));
-        float ax = in[9] * (0.5f * static_cast<float>(cam.width));
-        float ay = in[10] * (0.5f * static_cast<float>(cam.height));
-        statistics[id] = sqrtf(x * x + y * y);
-        statistics[n + id] = end > begin ? 1.0f : 0.0f;
-        absolute_statistics[id] = sqrtf(ax * ax + ay * ay);
-        absolute_statistics[n + id] = in[10];
-    }
+    write_statistics(cam, in, id, n, begin, end, statistics, absolute_statistics);
     if (screen) {
 #pragma unroll
         for (int k = 0; k < 11; ++k) {

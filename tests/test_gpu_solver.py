@@ -130,6 +130,30 @@ def test_fused_step_equals_gradients_then_adam(gsn, context, scenes, oracle, sh_
     np.testing.assert_array_equal(fused["second"].view(np.uint32), ev.view(np.uint32))
 
 
+@pytest.mark.parametrize("count", [33, 257, 20_011])
+def test_fused_step_at_counts_off_the_line_grid(gsn, context, scenes, oracle, count):
+    """A count that is not a multiple of 32 puts the parameter and moment rows off the 128-byte
+    line grid, where the fused pass uses ordinary instead of non-temporal accesses; 33 and 257 also
+    leave a partial last workgroup. The fused pass must equal the unfused gradients followed by
+    Adam, and its statistics the unfused pass's, bit for bit."""
+    params, cam, target = _scene_and_target(gsn, oracle, scenes, 61, count, 128, 3)
+    rng = np.random.default_rng(count)
+    first = rng.normal(0, 1e-4, size=params.shape).astype(np.float32)
+    second = rng.uniform(0, 1e-8, size=params.shape).astype(np.float32)
+    fused = solver_step(gsn, context, params, cam, target, 3, first=first, second=second, step=4)
+    run = GpuRun(gsn, context, params, cam, (0.0, 0.0, 0.0), 3)
+    got = run.gradients(target, screen=False)
+    c1, c2 = gsn.adam_corrections(4)
+    ep, em, ev = oracle.adam_f32(got["gradients"], params, first, second, 1e-4, c1, c2)
+    assert np.abs(got["gradients"]).max() > 0
+    np.testing.assert_array_equal(fused["params"].view(np.uint32), ep.view(np.uint32))
+    np.testing.assert_array_equal(fused["first"].view(np.uint32), em.view(np.uint32))
+    np.testing.assert_array_equal(fused["second"].view(np.uint32), ev.view(np.uint32))
+    np.testing.assert_array_equal(fused["statistics"].view(np.uint32), got["statistics"].view(np.uint32))
+    np.testing.assert_array_equal(fused["absolute_statistics"].view(np.uint32),
+                                  got["absolute_statistics"].view(np.uint32))
+
+
 def test_remap_history_bit_exact(gsn, context, oracle):
     rng = np.random.default_rng(4)
     previous, count = 700, 1000
