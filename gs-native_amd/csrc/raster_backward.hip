@@ -68,7 +68,12 @@ __device__ __forceinline__ int schedule_bin(int64_t records) {
 }
 
 // Heavy-first order of `units` work units; unit u owns records [offsets[u * stride],
-// offsets[min((u + 1) * stride, count)]). Run by the whole (single) workgroup.
+// offsets[min((u + 1) * stride, count)]). Run by one whole workgroup. The bins of the first
+// kScheduleItems * kScheduleThreads units are computed once into registers, with all their offset
+// loads in flight together; units past that recompute their bin. (Matching equal bins across the
+// wave to issue one LDS atomic per bin measured slower: the strided offset loads dominate.)
+constexpr int kScheduleItems = 16;
+
 template<typename Offset>
 __device__ void schedule_units(const Offset* __restrict__ offsets, int stride, int count, int units,
     int* __restrict__ order, int* histogram, int* cursor) {
@@ -77,11 +82,23 @@ __device__ void schedule_units(const Offset* __restrict__ offsets, int stride, i
         int64_t end = offsets[min(static_cast<int64_t>(u + 1) * stride, static_cast<int64_t>(count))];
         return schedule_bin(end - begin);
     };
+    int bins[kScheduleItems];
+#pragma unroll
+    for (int i = 0; i < kScheduleItems; ++i) {
+        const int u = threadIdx.x + i * kScheduleThreads;
+        bins[i] = u < units ? bin(u) : -1;
+    }
     for (int b = threadIdx.x; b < kScheduleBins; b += kScheduleThreads) {
         histogram[b] = 0;
     }
     __syncthreads();
-    for (int u = threadIdx.x; u < units; u += kScheduleThreads) {
+#pragma unroll
+    for (int i = 0; i < kScheduleItems; ++i) {
+        if (bins[i] >= 0) {
+            atomicAdd(&histogram[bins[i]], 1);
+        }
+    }
+    for (int u = threadIdx.x + kScheduleItems * kScheduleThreads; u < units; u += kScheduleThreads) {
         atomicAdd(&histogram[bin(u)], 1);
     }
     __syncthreads();
@@ -93,25 +110,35 @@ __device__ void schedule_units(const Offset* __restrict__ offsets, int stride, i
         }
     }
     __syncthreads();
-    for (int u = threadIdx.x; u < units; u += kScheduleThreads) {
+#pragma unroll
+    for (int i = 0; i < kScheduleItems; ++i) {
+        if (bins[i] >= 0) {
+            order[atomicAdd(&cursor[bins[i]], 1)] = threadIdx.x + i * kScheduleThreads;
+        }
+    }
+    for (int u = threadIdx.x + kScheduleItems * kScheduleThreads; u < units; u += kScheduleThreads) {
         order[atomicAdd(&cursor[bin(u)], 1)] = u;
     }
-    __syncthreads();
 }
 
-// Two schedules in one launch: the backward's tiles by record count, and the per-Gaussian pass's
-// workgroups (kGradientGroup consecutive Gaussians) by the length of their record run. Density
-// control appends its clones and split children, high-gradient Gaussians whose records are all
-// reached, at the end of the id range; in id order their workgroups, the longest of the pass,
-// were dispatched last and set its end (measured 1.3 -> 2.3 ms after one density step at 3M).
+// Two schedules, one per workgroup of the launch: the backward's tiles by record count, and the
+// per-Gaussian pass's workgroups (kGradientGroup consecutive Gaussians) by the length of their
+// record run. Density control appends its clones and split children, high-gradient Gaussians
+// whose records are all reached, at the end of the id range; in id order their workgroups, the
+// longest of the pass, were dispatched last and set its end (measured 1.3 -> 2.3 ms after one
+// density step at 3M).
 __global__ __launch_bounds__(kScheduleThreads) void tile_schedule_kernel(const int64_t* __restrict__ tile_offsets,
     int tile_count, int* __restrict__ order, const int* __restrict__ record_offsets, int gaussian_count,
     int* __restrict__ gaussian_order) {
     __shared__ int histogram[kScheduleBins];
     __shared__ int cursor[kScheduleBins];
-    schedule_units(tile_offsets, 1, tile_count, tile_count, order, histogram, cursor);
-    schedule_units(record_offsets, kGradientGroup, gaussian_count,
-        (gaussian_count + kGradientGroup - 1) / kGradientGroup, gaussian_order, histogram, cursor);
+    if (blockIdx.x == 0) {
+        schedule_units(tile_offsets, 1, tile_count, tile_count, order, histogram, cursor);
+    }
+    else {
+        schedule_units(record_offsets, kGradientGroup, gaussian_count,
+            (gaussian_count + kGradientGroup - 1) / kGradientGroup, gaussian_order, histogram, cursor);
+    }
 }
 
 // Plain stores: non-temporal stores of these scattered 16-byte pieces measured 2.3x the write
@@ -531,7 +558,7 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
         reinterpret_cast<const float4*>(conics), colors, reinterpret_cast<const int4*>(rects), record_offsets, count,
         splats);
     GSN_TRY(check_launch("pack_splats_kernel"));
-    tile_schedule_kernel<<<1, kScheduleThreads, 0, context->stream>>>(tile_offsets, static_cast<int>(tiles), order,
+    tile_schedule_kernel<<<2, kScheduleThreads, 0, context->stream>>>(tile_offsets, static_cast<int>(tiles), order,
         record_offsets, count, groups_order);
     *gaussian_order = groups_order;
     GSN_TRY(check_launch("tile_schedule_kernel"));
