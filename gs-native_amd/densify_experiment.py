@@ -6,7 +6,7 @@ Scenes are seeded mixtures of 64 anisotropic blobs: a ground-truth scene of `tru
 renders the targets; training starts from `init` points sampled from the same mixture with the
 reference initialisation (gaussian_splatting.cpp:95-128: KNN log-scales, opacity 0.1, SH DC from the
 point colour). Training follows the reference schedule (density control every 100 iterations after
-500, opacity resets) with a Gaussian-count cap. The plain rule stays the judge: the learned policy is
+500, opacity resets) with a Gaussian-count cap. The plain rule stays the arbiter: the learned policy is
 accepted only if its final held-out PSNR is within `tolerance` dB of the rule's.
 
 Usage: python gs-native_amd/densify_experiment.py [--quick] [--out result.json]

@@ -231,7 +231,7 @@ class Trainer:
         if self.features_enabled:
             self._reset_feature_state()
         if self.outcome_log is not None:
-            # Rule labels of this step are judged at the next one from what its outputs became.
+            # Rule labels of this step are measured at the next one from what its outputs became.
             self.pending = (decisions, offsets, list(t), count, features.numpy()[:count] if count else None)
         changes = (t[1] + 2 * t[3], count - t[0])
         self.history.append((self.completed, count, out_count, changes))
