@@ -76,14 +76,16 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
     uint32_t value_reg[kSortItems];
     int rank_reg[kSortItems];
     // All loads first (32 per lane in flight), then the ranking, which serialises on LDS counters.
+    // Each pass reads its input once: non-temporal loads (tile sort at 30M records 0.339 ->
+    // 0.285 ms; non-temporal stores of the scattered output measured 0.52 ms and stay ordinary).
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         int local = wave * kWaveChunk + j * 64 + lane;
         key_reg[j] = 0xffffffffu;
         value_reg[j] = 0u;
         if (local < valid) {
-            key_reg[j] = keys_in[tile_base + local];
-            value_reg[j] = values_in[tile_base + local];
+            key_reg[j] = __builtin_nontemporal_load(keys_in + tile_base + local);
+            value_reg[j] = __builtin_nontemporal_load(values_in + tile_base + local);
         }
     }
 #pragma unroll

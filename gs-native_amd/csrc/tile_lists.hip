@@ -175,8 +175,11 @@ __global__ __launch_bounds__(kEmitThreads) void emit_records_kernel(const uint32
                 int column = local - row * width;
                 uint32_t corner = s_corner[o];
                 uint32_t tile = static_cast<uint32_t>(((corner >> 16) + row) * tile_count_x + (corner & 0xffffu) + column);
-                record_tiles[base + p] = tile;
-                record_ids[base + p] = s_id[o];
+                // Written once and read once by the next sort pass: non-temporal stores keep the
+                // 240 MB of records (30M at 1080p) from displacing cached data (emit + tile sort
+                // 0.506 -> 0.473 ms).
+                __builtin_nontemporal_store(tile, record_tiles + base + p);
+                __builtin_nontemporal_store(s_id[o], record_ids + base + p);
                 atomicAdd(&s_digits[tile & digit_mask], 1);
             }
         }
