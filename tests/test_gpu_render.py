@@ -279,3 +279,27 @@ def test_image_metrics_match_oracle_loss(gsn, context, oracle, scenes):
     assert abs(metrics["loss"] - expected) < 1e-5 * max(1.0, abs(expected))
     mse = np.mean((np.clip(image, 0, 1).astype(np.float64) - target / 255.0) ** 2)
     assert abs(metrics["mse"] - mse) < 1e-6
+
+
+def test_stage_timing_mask_times_only_selected_stages(gsn, scenes):
+    """gsn_context_set_timing_mask records event pairs only around the selected stages; with
+    every stage selected, each stage of a render records one launch."""
+    context = gsn.Context()
+    params = scenes.config0_params(seed=5, count=2000, sh_degree=0)
+    cam = config0_camera(gsn)
+    try:
+        context.set_timing(True, stages=["raster_forward"])
+        GpuRun(gsn, context, params, cam)
+        times = context.stage_times()
+        assert times["raster_forward"][1] == 1 and times["raster_forward"][0] > 0.0
+        assert all(launches == 0 for name, (_, launches) in times.items() if name != "raster_forward")
+        context.set_timing(True)
+        GpuRun(gsn, context, params, cam)
+        times = context.stage_times()
+        for name in ("project", "scan", "depth_sort", "emit", "tile_sort", "ranges", "raster_forward"):
+            assert times[name][1] >= 1, name
+        context.set_timing(False)
+        GpuRun(gsn, context, params, cam)
+        assert all(launches == 0 for _, launches in context.stage_times().values())
+    finally:
+        context.close()
