@@ -127,12 +127,17 @@ __global__ __launch_bounds__(kEmitThreads) void emit_records_kernel(const uint32
             }
         }
         __syncthreads();
-        // Inclusive max-scan over the positions: thread t owns positions [8t, 8t + 8).
+        // Inclusive max-scan over the positions: thread t owns positions [kEmitItems t, kEmitItems (t + 1)).
+        static_assert(kEmitItems % 4 == 0, "positions are read as int4");
         int v[kEmitItems];
-        const int4 lo = *reinterpret_cast<const int4*>(&s_owner[kEmitItems * t]);
-        const int4 hi = *reinterpret_cast<const int4*>(&s_owner[kEmitItems * t + 4]);
-        v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
-        v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
+#pragma unroll
+        for (int q = 0; q < kEmitItems / 4; ++q) {
+            const int4 quad = *reinterpret_cast<const int4*>(&s_owner[kEmitItems * t + 4 * q]);
+            v[4 * q] = quad.x;
+            v[4 * q + 1] = quad.y;
+            v[4 * q + 2] = quad.z;
+            v[4 * q + 3] = quad.w;
+        }
 #pragma unroll
         for (int i = 1; i < kEmitItems; ++i) {
             v[i] = max(v[i], v[i - 1]);
@@ -158,11 +163,11 @@ __global__ __launch_bounds__(kEmitThreads) void emit_records_kernel(const uint32
         for (int w = 0; w < wave; ++w) {
             carry = max(carry, s_wave_max[w]);
         }
-        int4 out_lo, out_hi;
-        out_lo.x = max(v[0], carry); out_lo.y = max(v[1], carry); out_lo.z = max(v[2], carry); out_lo.w = max(v[3], carry);
-        out_hi.x = max(v[4], carry); out_hi.y = max(v[5], carry); out_hi.z = max(v[6], carry); out_hi.w = max(v[7], carry);
-        *reinterpret_cast<int4*>(&s_owner[kEmitItems * t]) = out_lo;
-        *reinterpret_cast<int4*>(&s_owner[kEmitItems * t + 4]) = out_hi;
+#pragma unroll
+        for (int q = 0; q < kEmitItems / 4; ++q) {
+            *reinterpret_cast<int4*>(&s_owner[kEmitItems * t + 4 * q]) = make_int4(max(v[4 * q], carry),
+                max(v[4 * q + 1], carry), max(v[4 * q + 2], carry), max(v[4 * q + 3], carry));
+        }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kEmitItems; ++j) {
