@@ -2,21 +2,36 @@
 """Training throughput of the gfx950 Gaussian-splatting path on one MI355X.
 
 Workload: 3M Gaussians (seeded 64-blob mixture, scales from the GPU 3-NN initialisation),
-1920x1080, SH degree 3, 300 cameras on a radius-5 sphere with targets rendered from the ground
-truth and resident in HBM. One step is one full training iteration of the reference schedule
-(render, L1+SSIM gradient, backward, Adam, statistics, density control at its 100-iteration
-cadence) starting at iteration 3000. The CPU baseline is the oracle timing one full iteration of
-the same workload on one host thread. Prints one JSON line.
+1920x1080, SH degree 3, 300 cameras on a radius-5 sphere, targets rendered from the ground truth
+and resident in HBM. The trained state is a fixed 3M-Gaussian benchmark state (the ground truth
+perturbed by N(0, 0.01)), entered at a chosen iteration of the reference's training schedule. One
+step is one full training iteration of that schedule (render, L1+SSIM gradient, backward, Adam,
+statistics, and density control / opacity caps where the schedule has them).
+
+Sequence (one process):
+  1. warm-up (W iterations), then a stage pass (S iterations) with an event pair around every
+     stage and the gradient pass unfused from Adam (bit-identical results) so that Adam has a
+     time of its own; the raster forward then counts its (pixel, splat) evaluations on the stage
+     pass's cameras (the VALU roofline's operation count);
+  2. the timed window: K iterations placed so that the density step at iteration 3100 falls in
+     its first fifth (the count then leaves the 32-Gaussian grid for the rest of the window, as
+     it does for most iterations of a real run); events only around the dominant stage and
+     density control;
+  3. a second figure from a fresh copy of the same state entered at iteration 2999, so that the
+     reference's iteration-3000 opacity cap (0.01) is applied before K timed iterations
+     (the rasteriser then reaches far more records per tile);
+  4. the CPU baseline (the oracle, C + OpenMP).
+Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
 import importlib.util
 import json
-import re
 import os
 import pathlib
-import platform
+import re
+import statistics
 import sys
 import time
 
@@ -25,6 +40,11 @@ import numpy as np
 REPO = pathlib.Path(__file__).resolve().parent
 METRIC = "iters/s + fwd/bwd ms @3M Gaussians 1080p; learned vs rule densify at equal PSNR"
 HBM_PEAK_GBS = 8000.0
+FP32_VALU_PEAK_TFLOPS = 157.3  # published MI355X FP32 vector peak
+FORWARD_FLOP_PER_PAIR = 20     # per evaluated (pixel, splat) pair, plus one exp
+BACKWARD_FLOP_PER_PAIR = 60
+DENSITY_ITERATION = 3100       # a density step of the schedule (every 100 iterations after 500)
+RESET_ITERATION = 3000         # the schedule's 0.01 opacity cap (every 3000 iterations)
 
 
 def load_package():
@@ -39,51 +59,63 @@ def load_package():
     return module
 
 
+def load_module(name: str, path: pathlib.Path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    module = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(module)
+    return module
+
+
 def log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(n: int, v: float, r: float, p: int, t: int, k: int = 16) -> dict:
-    """Compulsory HBM bytes per launch of each stage (one iteration)."""
+def algorithmic_bytes(*, n: float, v: float, r: float, p: int, t: int, k: int = 16) -> dict:
+    """Compulsory HBM bytes of one launch of each stage of this build, from its inputs and outputs:
+    n Gaussians, v visible, r (tile, Gaussian) records, p pixels, t tiles, k SH coefficients per
+    channel. Sort stages count one read and one write of their (key, id) pairs (the sort's lower
+    bound; an LSD pass moves about that much again per pass). The tile records are 8 bytes (tile id,
+    Gaussian id). The per-Gaussian gradient pass with Adam fused in never writes or re-reads a
+    gradient buffer: its floor is the Adam rows (params, both moments) read and written (1392n), the
+    statistics written (16n), the run offsets (4n) and one flag byte per record (r); the reached
+    records' 48-byte partials come on top (data-dependent, not counted)."""
     return {
         "project": 16 * n + v * (28 + 12 * k + 56),
         "scan": 8 * n,
-        "depth_sort": 8 * n,
-        "emit": 24 * v + 12 * r,
-        "tile_sort": 24 * r,
-        "ranges": 8 * r + 8 * (t + 1),
+        "depth_sort": 16 * n,
+        "emit": 24 * v + 8 * r,
+        "tile_sort": 16 * r,
+        "ranges": 4 * r + 8 * (t + 1),
         "raster_forward": 40 * r + 12 * p + 8 * (t + 1),
         "image_gradients": 27 * p,
         "raster_backward": 40 * r + 24 * p + 44 * v,
-        "gaussian_gradients": 32 * n + 500 * v + 1624 * n,
+        "gaussian_gradients": 1412 * n + r,
+        "gaussian_gradients_unfused": 484 * n + r,  # params read, gradients written, statistics, offsets, flags
+        "adam": 1624 * n,                           # params, moments read + written, gradients read
+        "accumulate": 24 * n,
     }
 
 
-def fused_floor_bytes(n: int, r: float) -> dict:
-    """Compulsory bytes of the fused gradient + Adam pass itself, which never writes or re-reads a
-    58-row gradient buffer: Adam's parameter and moment rows read and written (1392N), the
-    statistics (16N), the run offsets (4N) and one flag byte per record (R). The reached records'
-    48-byte partials come on top (data-dependent, not counted)."""
-    return {"gaussian_gradients": 1412 * n + r}
+def unfused_equivalent_bytes(n: float, v: float) -> float:
+    """The per-Gaussian pass priced as the reference's separate kernels: statistics 32n, chain
+    rule 500v, Adam 1624n (including the 58-row gradient buffer the fused pass never moves)."""
+    return 32 * n + 500 * v + 1624 * n
 
 
 def profiled_value(key: str, stage: str):
-    """A per-kernel figure recorded by a committed rocprofv3 summary of this workload under
-    profiles/ ("<key> <stage> = <value>" lines), or None: traffic_bytes_per_launch (FETCH_SIZE +
-    WRITE_SIZE scaled by a plain copy of known size) and valu_share (SQ_INSTS_VALU issue share)."""
+    """A per-kernel figure from a committed rocprofv3 summary of this workload under profiles/
+    ("<key> <stage> = <value>" lines) and the file it came from, or (None, None)."""
     for path in sorted((REPO / "profiles").glob("*.md")):
         for line in path.read_text().splitlines():
             match = re.match(r"^(\w+)\s+(\w+)\s*=\s*([0-9.eE+-]+)$", line.strip())
             if match and match.group(1) == key and match.group(2) == stage:
-                return float(match.group(3))
-    return None
+                return float(match.group(3)), f"profiles/{path.name}"
+    return None, None
 
 
 def build_workload(gsn, context, gaussians, width, height, cameras, seed=0, scene="blobs", camera_radius=5.0,
                    hemisphere=False):
-    scenes_spec = importlib.util.spec_from_file_location("gs_native_amd.scenes", REPO / "gs-native_amd" / "scenes.py")
-    scenes = importlib.util.module_from_spec(scenes_spec)
-    scenes_spec.loader.exec_module(scenes)
+    scenes = load_module("gs_native_amd.scenes", REPO / "gs-native_amd" / "scenes.py")
     if scene == "shell":
         truth = scenes.shell_plane_scene(gaussians, seed=seed, sh_degree=3)
     else:
@@ -98,7 +130,7 @@ def build_workload(gsn, context, gaussians, width, height, cameras, seed=0, scen
     extent = 1.1 * float(np.linalg.norm(centres - centres.mean(0), axis=1).max())
     device_truth = gsn.DeviceBuffer.from_numpy(truth)
     image = gsn.DeviceBuffer((height, width, 3), np.float32)
-    targets, visible, records = [], [], []
+    targets, visible = [], []
     projection = None
     for cam in cams:
         projection = gsn.render(context, device_truth, gaussians, cam, (0.0, 0.0, 0.0), image, 3, projection)
@@ -110,134 +142,264 @@ def build_workload(gsn, context, gaussians, width, height, cameras, seed=0, scen
         projection = gsn.render(context, device_params, gaussians, cam, (0.0, 0.0, 0.0), image, 3, projection)
         counts = np.diff(projection.record_offsets.numpy())
         visible.append(int((counts > 0).sum()))
-        records.append(int(projection.record_count))
     del device_params
-    return params, cams, targets, extent, visible, records
+    return params, cams, targets, extent, visible
 
 
-def cpu_baseline(params: np.ndarray, camera, target: np.ndarray) -> dict:
-    """The oracle (C, one thread) on one full training iteration of this same workload, first
-    camera: projection, render, L1+SSIM gradient, replay backward, chain rule (FP64) and Adam
-    (FP32), all N Gaussians at the full image size. One timed run: it takes tens of seconds."""
+def host_cpu() -> dict:
+    model = None
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model or "unknown", "nproc": nproc}
+
+
+def cpu_baseline(gsn, params: np.ndarray, camera, target: np.ndarray, runs: int) -> dict:
+    """The oracle (C, OpenMP) timing one full training iteration: FP64 projection, render,
+    L1+SSIM gradient, replay backward and chain rule, FP32 Adam (the reference has no CPU training
+    path; the oracle restates its math). Two samples:
+      * this workload (all Gaussians, full image, the first camera): median of `runs` after one
+        warm-up on all the host threads the box grants, plus one single-thread run;
+      * the 10,000-Gaussian 256x256 SH-0 iteration (perturbed u8 target, position LR 1.6e-4):
+        median of `runs` after one warm-up, single thread and all threads."""
     sys.path.insert(0, str(REPO))
     import oracle
-    first = np.zeros_like(params)
-    second = np.zeros_like(params)
-    sh_degree = 3
-    start = time.perf_counter()
-    projected = oracle.project_f64(params, camera, sh_degree)
-    image, _ = oracle.render_f64(camera, projected, (0.0, 0.0, 0.0))
-    image_grad = oracle.image_gradients_f64(image, target)
-    back = oracle.backward_f64(camera, projected, image, image_grad)
-    mapped = oracle.map_f64(params, camera, sh_degree, back["screen"], projected["visible"])
-    oracle.adam_f32(mapped["gradients"].astype(np.float32), params, first, second, 1.6e-4, 10.0, 1000.0)
-    elapsed = time.perf_counter() - start
+
+    scenes = load_module("gs_native_amd.scenes", REPO / "gs-native_amd" / "scenes.py")
+    cpu = host_cpu()
+    threads = max(1, min(cpu["nproc"] or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+
+    def iteration(p, cam, tgt, sh_degree):
+        first = np.zeros_like(p)
+        second = np.zeros_like(p)
+        start = time.perf_counter()
+        projected = oracle.project_f64(p, cam, sh_degree)
+        image, _ = oracle.render_f64(cam, projected, (0.0, 0.0, 0.0))
+        image_grad = oracle.image_gradients_f64(image, tgt)
+        back = oracle.backward_f64(cam, projected, image, image_grad)
+        mapped = oracle.map_f64(p, cam, sh_degree, back["screen"], projected["visible"])
+        oracle.adam_f32(mapped["gradients"].astype(np.float32), p, first, second, 1.6e-4, 10.0, 1000.0)
+        return time.perf_counter() - start
+
+    def median_of(p, cam, tgt, sh_degree, count):
+        iteration(p, cam, tgt, sh_degree)
+        return statistics.median(iteration(p, cam, tgt, sh_degree) for _ in range(count))
+
+    oracle.set_threads(threads)
+    workload_all = median_of(params, camera, target, 3, runs)
+    oracle.set_threads(1)
+    workload_single = iteration(params, camera, target, 3)
+    # 10,000 Gaussians at 256x256, SH 0; target rendered from a copy perturbed by N(0, 0.01).
+    small = scenes.config0_params(seed=0, count=10_000, sh_degree=0)
+    focal = 128.0 / np.tan(np.radians(30.0))
+    small_camera = gsn.Camera.make(256, 256, focal, focal, 127.5, 127.5, None, [0.0, 0.0, 3.0])
+    truth = oracle.render_f64(small_camera, oracle.project_f64(scenes.perturb(small, 1, 0.01), small_camera, 0),
+                              (0.0, 0.0, 0.0))[0]
+    small_target = np.clip(np.rint(truth * 255.0), 0, 255).astype(np.uint8)
+    small_single = median_of(small, small_camera, small_target, 0, runs)
+    oracle.set_threads(threads)
+    small_all = median_of(small, small_camera, small_target, 0, runs)
+    oracle.set_threads(1)
     count = params.shape[1]
-    return {"value": 1.0 / elapsed, "unit": "iters/s", "cores": 1, "kind": "port",
-            "sample": f"one full training iteration of this workload ({count} Gaussians, {camera.width}x{camera.height}, "
-                      f"SH 3, camera 0): oracle FP64 projection/render/L1+SSIM gradient/replay backward/chain rule + "
-                      f"FP32 Adam, one timed run ({elapsed:.1f} s), single thread on "
-                      f"{platform.processor() or platform.machine()}"}
+    return {
+        "value": 1.0 / workload_all,
+        "unit": "iters/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one full training iteration of this workload ({count} Gaussians, {camera.width}x{camera.height}, "
+                  f"SH 3, camera 0): oracle FP64 projection / render / L1+SSIM gradient / replay backward / chain "
+                  f"rule + FP32 Adam, C with OpenMP on {threads} threads, median of {runs} runs after one warm-up "
+                  f"({workload_all:.2f} s); host {cpu['model']}, nproc {cpu['nproc']}",
+        "single_thread_value": 1.0 / workload_single,
+        "single_thread_s": workload_single,
+        "host": cpu,
+        "config0_iteration_ms": {
+            "workload": "10,000 Gaussians, 256x256, SH 0, one camera at distance 3, perturbed u8 target, "
+                        "render + L1+SSIM gradient + backward + chain rule + Adam (position LR 1.6e-4)",
+            "single_thread_median": 1000.0 * small_single,
+            "all_threads_median": 1000.0 * small_all,
+            "threads": threads,
+            "runs": runs,
+        },
+    }
+
+
+def stage_rates(stages: dict, bytes_of: dict, label_of=None) -> dict:
+    """Per-stage ms per launch and algorithmic GB/s; flags any figure above the HBM peak (a wrong
+    byte count rather than a fast kernel)."""
+    out = {}
+    for name, (ms, launches) in stages.items():
+        if not launches:
+            continue
+        entry = {"ms_per_launch": ms / launches, "launches": launches, "ms_total": ms}
+        key = label_of(name) if label_of else name
+        if key in bytes_of:
+            gbs = bytes_of[key] / (ms / launches * 1e-3) / 1e9
+            entry["algorithmic_bytes"] = bytes_of[key]
+            entry["hbm_gbs_algorithmic"] = gbs
+            if gbs > HBM_PEAK_GBS:
+                entry["exceeds_hbm_peak"] = True
+                log(f"warning: stage {name} claims {gbs:.0f} GB/s algorithmic, above the {HBM_PEAK_GBS:.0f} GB/s peak")
+        out[name] = entry
+    return out
 
 
 def main() -> int:
-    parser = argparse.ArgumentParser(description=__doc__)
+    parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=100)
+    parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=5)
     parser.add_argument("--gaussians", type=int, default=3_000_000)
     parser.add_argument("--width", type=int, default=1920)
     parser.add_argument("--height", type=int, default=1080)
     parser.add_argument("--cameras", type=int, default=300)
-    parser.add_argument("--start-iteration", type=int, default=3000)
+    parser.add_argument("--first-timed-iteration", type=int, default=None,
+                        help="first iteration of the timed window (default: the density step at 3100 falls in the "
+                             "window's first fifth)")
     parser.add_argument("--scene", choices=["blobs", "shell"], default="blobs",
                         help="blobs: seeded 64-blob mixture; shell: radius-2 shell plus ground plane")
     parser.add_argument("--camera-radius", type=float, default=5.0)
     parser.add_argument("--hemisphere", action="store_true", help="cameras on the upper hemisphere only")
     parser.add_argument("--stage-steps", type=int, default=20,
-                        help="iterations of the separate pass that times every stage (before the timed steps)")
+                        help="iterations of the stage pass (before the timed window)")
+    parser.add_argument("--no-post-reset", action="store_true", help="skip the figure after the opacity cap")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--cpu-runs", type=int, default=5)
     parser.add_argument("--calibration-copy-mib", type=int, default=0,
                         help="after the timed steps, run one device-to-device copy of this many MiB (a plain copy "
                              "kernel of known size for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the training step runs on a "
-              f"single MI355X; multi-GPU sharding is not implemented).", file=sys.stderr)
+              f"single MI355X; sharding across GPUs is a design note, not built).", file=sys.stderr)
+        return 2
+    if args.steps < 1 or args.warmup < 0 or args.stage_steps < 0:
+        print("bench.py: --steps must be >= 1, --warmup and --stage-steps >= 0", file=sys.stderr)
         return 2
     gsn = load_package()
-    trainer_spec = importlib.util.spec_from_file_location("gs_native_amd.trainer", REPO / "gs-native_amd" / "trainer.py")
-    trainer_module = importlib.util.module_from_spec(trainer_spec)
-    trainer_spec.loader.exec_module(trainer_module)
+    trainer_module = load_module("gs_native_amd.trainer", REPO / "gs-native_amd" / "trainer.py")
     context = gsn.Context()
     log(f"device {gsn.device_name()}")
     log(f"building workload: {args.gaussians} Gaussians, {args.width}x{args.height}, {args.cameras} cameras")
-    params, cameras, targets, extent, visible, records = build_workload(
+    params, cameras, targets, extent, visible = build_workload(
         gsn, context, args.gaussians, args.width, args.height, args.cameras, scene=args.scene,
         camera_radius=args.camera_radius, hemisphere=args.hemisphere)
-    log(f"workload ready: mean visible {np.mean(visible):.0f}, mean records {np.mean(records):.0f}")
-    trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=args.start_iteration)
+    log(f"workload ready: mean visible {np.mean(visible):.0f}")
+    k_steps = args.steps
+    first_timed = args.first_timed_iteration or max(RESET_ITERATION + 1, DENSITY_ITERATION - k_steps // 5)
+    stage_steps = args.stage_steps
+    start = first_timed - 1 - stage_steps - args.warmup
+    if start < RESET_ITERATION:
+        print("bench.py: the warm-up and stage pass must start after iteration 3000 (SH degree 3, after the opacity "
+              "cap); lower --warmup / --stage-steps or raise --first-timed-iteration", file=sys.stderr)
+        return 2
+    p = args.width * args.height
+    t = ((args.width + 15) // 16) * ((args.height + 15) // 16)
+
+    trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=start)
     for _ in range(args.warmup):
         trainer.iteration()
     context.synchronize()
-    log(f"warm-up done ({args.warmup} iterations), gaussians {trainer.count}")
-    p = args.width * args.height
-    t = ((args.width + 15) // 16) * ((args.height + 15) // 16)
-    # Stage breakdown: a separate pass with an event pair around every stage. Each event record
-    # leaves the stream idle for a few microseconds, so the timed steps below record events around
-    # the dominant stage only.
-    stage_steps = max(args.stage_steps, 0)
-    per_stage = {}
-    dominant = "gaussian_gradients"
+    log(f"warm-up done: iterations {start + 1}-{trainer.completed}, gaussians {trainer.count}")
+
+    # 1. Stage pass: every stage timed; Adam separate from the gradient pass.
+    per_stage, raster = {}, None
     if stage_steps:
+        trainer.fused = False
         context.set_timing(True)
-        counts = [0, 0, 0]
+        sums = {"n": 0.0, "v": 0.0, "r": 0.0}
+        stage_cameras = []
+        stage_first = trainer.completed + 1
         for _ in range(stage_steps):
-            counts[0] += trainer.count
-            counts[1] += trainer.iteration()
-            counts[2] += visible[trainer.last_camera]
+            sums["n"] += trainer.count
+            sums["r"] += trainer.iteration()
+            sums["v"] += visible[trainer.last_camera]
+            stage_cameras.append(trainer.last_camera)
         stages = context.stage_times()
         context.set_timing(False)
-        stage_bytes = algorithmic_bytes(*(c / stage_steps for c in counts), p, t)
-        for name, (ms, launches) in stages.items():
-            if launches:
-                per_stage[name] = {"ms_per_iteration": ms / stage_steps, "launches": launches}
-                if name in stage_bytes:
-                    per_stage[name]["hbm_gbs_algorithmic"] = stage_bytes[name] / (ms / launches * 1e-3) / 1e9
-        candidates = {name: stages[name][0] for name in stage_bytes if stages[name][1]}
-        dominant = max(candidates, key=candidates.get)
-        log(f"stage pass done ({stage_steps} iterations), dominant stage {dominant}")
-    context.set_timing(True, stages=[dominant])
+        trainer.fused = True
+        trainer.gradients = None
+        mean = {key: value / stage_steps for key, value in sums.items()}
+        bytes_of = algorithmic_bytes(n=mean["n"], v=mean["v"], r=mean["r"], p=p, t=t)
+        per_stage = stage_rates(stages, bytes_of,
+                                lambda name: "gaussian_gradients_unfused" if name == "gaussian_gradients" else name)
+        # Raster operation count: forward renders of the stage pass's cameras with pair counting.
+        context.count_raster_pairs(True)
+        context.raster_pairs()
+        for index in stage_cameras:
+            gsn.render(context, trainer.params, trainer.count, cameras[index], trainer.background, trainer.image,
+                       trainer.sh_degree)
+        pairs = context.raster_pairs() / len(stage_cameras)
+        context.count_raster_pairs(False)
+        fwd_ms = per_stage["raster_forward"]["ms_per_launch"]
+        bwd_ms = per_stage["raster_backward"]["ms_per_launch"]
+        fwd_tflops = FORWARD_FLOP_PER_PAIR * pairs / (fwd_ms * 1e-3) / 1e12
+        bwd_tflops = BACKWARD_FLOP_PER_PAIR * pairs / (bwd_ms * 1e-3) / 1e12
+        raster_bytes = bytes_of["raster_forward"] + bytes_of["raster_backward"]
+        raster = {
+            "evaluated_pairs_per_iteration": pairs,
+            "records_per_iteration": mean["r"],
+            "forward": {"ms": fwd_ms, "flop_per_pair": FORWARD_FLOP_PER_PAIR, "exp_per_pair": 1,
+                        "tflops": fwd_tflops, "valu_frac": fwd_tflops / FP32_VALU_PEAK_TFLOPS},
+            "backward": {"ms": bwd_ms, "flop_per_pair": BACKWARD_FLOP_PER_PAIR, "exp_per_pair": 1,
+                         "tflops": bwd_tflops, "valu_frac": bwd_tflops / FP32_VALU_PEAK_TFLOPS},
+            "valu_peak_tflops": FP32_VALU_PEAK_TFLOPS,
+            "fwd_bwd_hbm_frac": raster_bytes / ((fwd_ms + bwd_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "fwd_bwd_algorithmic_bytes": raster_bytes,
+            "note": "pairs counted by the forward (pixel not done, band kept by the conservative mask); the backward "
+                    "replays the same pairs. fwd_bwd_hbm_frac is the north-star 'raster at >=50% of the HBM roofline' "
+                    "figure: compulsory bytes (40R + 12P + 8(T+1) forward, 40R + 24P + 44V backward) over the two "
+                    "kernels' time",
+        }
+        per_stage_note = (f"iterations {stage_first}-{trainer.completed} with an event pair around every stage; the "
+                          f"gradient pass runs unfused there (writes the 58-row gradients, then the row-major Adam "
+                          f"kernel: bit-identical results) so that Adam has its own time")
+        log(f"stage pass done: iterations {stage_first}-{trainer.completed}, {pairs:.3e} raster pairs per view")
+    else:
+        per_stage_note = "no stage pass"
+
+    # 2. Timed window.
+    while trainer.completed < first_timed - 1:
+        trainer.iteration()
+    dominant = "gaussian_gradients"
+    context.set_timing(True, stages=[dominant, "density"])
     start_count = trainer.count
-    first_timed = trainer.completed + 1
-    record_total = 0
-    visible_total = 0
-    gaussian_total = 0
+    history_before = len(trainer.history)
+    totals = {"n": 0.0, "v": 0.0, "r": 0.0}
+    context.synchronize()
     begin = time.perf_counter()
-    for step in range(args.steps):
-        gaussian_total += trainer.count
-        record_total += trainer.iteration()
-        visible_total += visible[trainer.last_camera]
+    for step in range(k_steps):
+        totals["n"] += trainer.count
+        totals["r"] += trainer.iteration()
+        totals["v"] += visible[trainer.last_camera]
         if (step + 1) % 25 == 0:
-            log(f"step {step + 1}/{args.steps}")
+            log(f"step {step + 1}/{k_steps}")
     context.synchronize()
     elapsed = time.perf_counter() - begin
     timed = context.stage_times()
     context.set_timing(False)
-    ms_per_step = 1000.0 * elapsed / args.steps
-    n_mean = gaussian_total / args.steps
-    r_mean = record_total / args.steps
-    v_mean = visible_total / args.steps
-    bytes_per_launch = algorithmic_bytes(n_mean, v_mean, r_mean, p, t)
+    last_timed = trainer.completed
+    log(f"timed window done: iterations {first_timed}-{last_timed}, {k_steps / elapsed:.1f} it/s")
+    n_mean, v_mean, r_mean = (totals[key] / k_steps for key in ("n", "v", "r"))
+    window_bytes = algorithmic_bytes(n=n_mean, v=v_mean, r=r_mean, p=p, t=t)
     launches = timed[dominant][1]
     avg_ms = timed[dominant][0] / launches
-    achieved = bytes_per_launch[dominant] / (avg_ms * 1e-3) / 1e9
+    achieved = window_bytes[dominant] / (avg_ms * 1e-3) / 1e9
+    density_ms, density_launches = timed["density"]
+    density_steps = trainer.history[history_before:]
+    traffic, traffic_source = profiled_value("traffic_bytes_per_launch", dominant)
+    ms_per_step = 1000.0 * elapsed / k_steps
     result = {
         "metric": METRIC,
-        "value": args.steps / elapsed,
+        "value": k_steps / elapsed,
         "unit": "iters/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": k_steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
@@ -249,37 +411,40 @@ def main() -> int:
             "workload": f"{args.gaussians:,}-Gaussian "
                         f"{'shell + ground-plane' if args.scene == 'shell' else '64-blob'} scene, "
                         f"{args.width}x{args.height}, SH degree 3, {args.cameras} cameras "
-                        f"({'hemisphere' if args.hemisphere else 'sphere'}, radius {args.camera_radius:g}), "
-                        f"training iterations {first_timed}-{first_timed + args.steps - 1} of the reference schedule",
+                        f"({'hemisphere' if args.hemisphere else 'sphere'}, radius {args.camera_radius:g}); "
+                        f"fixed benchmark state (ground truth perturbed by N(0, 0.01), no opacity cap applied) "
+                        f"entered at iteration {start}; timed iterations {first_timed}-{last_timed} of the "
+                        f"reference schedule, including the density step at {DENSITY_ITERATION}",
             "gaussians_start": start_count,
             "gaussians_end": trainer.count,
+            "mean_gaussians": n_mean,
             "mean_records": r_mean,
             "mean_visible": v_mean,
-            "density_steps": [h for h in trainer.history],
+            "density_steps": [list(h) for h in density_steps],
         },
+        "density_control_ms": density_ms if density_launches else None,
         "stages": per_stage,
-        "stage_pass": {"iterations": stage_steps, "note": "per-stage times from a separate pass before the timed "
-                       "steps, with an event pair around every stage; the timed steps record events around "
-                       "the dominant stage only"},
+        "stage_pass": {"iterations": stage_steps, "note": per_stage_note},
+        "raster": raster,
         "roofline": {
-            "kernel": dominant,
+            "kernel": "gaussian_gradients_kernel (per-Gaussian gradients + chain rule + Adam, fused)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_source,
             "avg_launch_ms": avg_ms,
-            "algorithmic_bytes_per_launch": bytes_per_launch[dominant],
+            "launches": launches,
+            "algorithmic_bytes_per_launch": window_bytes[dominant],
+            "bytes_note": "fused floor 1412N + R (Adam rows read + written, statistics, run offsets, record flags); "
+                          "the partials of reached records are extra",
+            "unfused_equivalent_bytes_per_launch": unfused_equivalent_bytes(n_mean, v_mean),
+            "unfused_equivalent_frac": unfused_equivalent_bytes(n_mean, v_mean) / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": None,
     }
-    floor = fused_floor_bytes(n_mean, r_mean).get(dominant)
-    if floor is not None:
-        result["roofline"]["fused_floor_bytes_per_launch"] = floor
-        result["roofline"]["frac_of_fused_floor"] = floor / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-    result["roofline"]["traffic"] = profiled_value("traffic_bytes_per_launch", dominant)
-    result["roofline"]["valu_share"] = profiled_value("valu_share", dominant)
     if args.calibration_copy_mib > 0:
         size = args.calibration_copy_mib << 20
         source = gsn.DeviceBuffer((size,), np.uint8)
@@ -288,9 +453,36 @@ def main() -> int:
         context.synchronize()
         log(f"calibration copy of {size} bytes done")
         del source, destination
+    del trainer
+
+    # 3. The same state entered just before the iteration-3000 opacity cap.
+    if not args.no_post_reset:
+        reset_trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=RESET_ITERATION - 1)
+        reset_trainer.iteration()  # iteration 3000: the 0.01 opacity cap is applied after it
+        for _ in range(args.warmup):
+            reset_trainer.iteration()
+        context.synchronize()
+        first_reset = reset_trainer.completed + 1
+        records = 0
+        begin = time.perf_counter()
+        for _ in range(k_steps):
+            records += reset_trainer.iteration()
+        context.synchronize()
+        reset_elapsed = time.perf_counter() - begin
+        result["after_opacity_cap"] = {
+            "value": k_steps / reset_elapsed,
+            "ms_per_step": 1000.0 * reset_elapsed / k_steps,
+            "iterations": f"{first_reset}-{reset_trainer.completed}",
+            "mean_records": records / k_steps,
+            "note": f"same state entered at iteration {RESET_ITERATION - 1}: the schedule's 0.01 opacity cap is "
+                    f"applied after iteration {RESET_ITERATION}, then {args.warmup} warm-up and {k_steps} timed "
+                    f"iterations",
+        }
+        log(f"after-cap window done: {k_steps / reset_elapsed:.1f} it/s")
+        del reset_trainer
     if not args.no_cpu_baseline:
-        log("timing the CPU baseline: one oracle iteration of this workload (tens of seconds)")
-        result["cpu_baseline"] = cpu_baseline(params, cameras[0], targets[0].numpy())
+        log("timing the CPU baseline (oracle, C + OpenMP)")
+        result["cpu_baseline"] = cpu_baseline(gsn, params, cameras[0], targets[0].numpy(), args.cpu_runs)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -138,10 +138,12 @@ _PROTOTYPES = {
     "gsn_context_set_timing": [_P, _I],
     "gsn_context_set_timing_mask": [_P, ctypes.c_uint],
     "gsn_context_stage_times": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), _I],
+    "gsn_context_count_raster_pairs": [_P, _I],
+    "gsn_context_raster_pairs": [_P, ctypes.POINTER(ctypes.c_ulonglong)],
 }
 
 STAGES = ("project", "scan", "depth_sort", "emit", "tile_sort", "ranges", "raster_forward", "image_gradients",
-          "raster_backward", "gaussian_gradients", "adam", "density", "knn")
+          "raster_backward", "gaussian_gradients", "adam", "density", "knn", "accumulate")
 
 _RESTYPES = {
     "gsn_context_destroy": None,
@@ -285,6 +287,16 @@ class Context:
         launches = (ctypes.c_int * len(STAGES))()
         call("gsn_context_stage_times", self._handle, ms, launches, len(STAGES))
         return {name: (ms[k], launches[k]) for k, name in enumerate(STAGES)}
+
+    def count_raster_pairs(self, enabled: bool) -> None:
+        """While enabled, the raster forward counts the (pixel, splat) pairs it evaluates."""
+        call("gsn_context_count_raster_pairs", self._handle, int(enabled))
+
+    def raster_pairs(self) -> int:
+        """Evaluated (pixel, splat) pairs since the last read (clears the counter)."""
+        value = ctypes.c_ulonglong(0)
+        call("gsn_context_raster_pairs", self._handle, ctypes.byref(value))
+        return int(value.value)
 
     def close(self) -> None:
         if self._handle.value:

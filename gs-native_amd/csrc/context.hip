@@ -120,6 +120,9 @@ void gsn_context_destroy(gsn_context* context) {
     if (context->readback_event) {
         (void)hipEventDestroy(context->readback_event);
     }
+    if (context->raster_pairs) {
+        (void)hipFree(context->raster_pairs);
+    }
     for (int k = 0; k < 2 * context->event_capacity; ++k) {
         if (context->events[k]) {
             (void)hipEventDestroy(context->events[k]);
@@ -381,5 +384,37 @@ extern "C" int gsn_context_stage_times(gsn_context* context, double* millisecond
         }
     }
     context->event_used = 0;
+    return GSN_OK;
+}
+
+extern "C" int gsn_context_count_raster_pairs(gsn_context* context, int enabled) {
+    if (!context) {
+        return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_context_count_raster_pairs: null context");
+    }
+    if (enabled && !context->raster_pairs) {
+        void* counter = nullptr;
+        GSN_HIP(hipMalloc(&counter, sizeof(unsigned long long)));
+        context->raster_pairs = static_cast<unsigned long long*>(counter);
+        GSN_HIP(hipMemsetAsync(context->raster_pairs, 0, sizeof(unsigned long long), context->stream));
+    }
+    context->count_raster_pairs = enabled != 0;
+    return GSN_OK;
+}
+
+extern "C" int gsn_context_raster_pairs(gsn_context* context, unsigned long long* pairs) {
+    if (!context || !pairs) {
+        return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_context_raster_pairs: invalid arguments");
+    }
+    *pairs = 0;
+    if (!context->raster_pairs) {
+        return GSN_OK;
+    }
+    // Bytes 128.. of the pinned buffer: the record-count and density readbacks use its start.
+    void* host = static_cast<char*>(context->pinned) + 128;
+    GSN_HIP(hipMemcpyAsync(host, context->raster_pairs, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+        context->stream));
+    GSN_HIP(hipMemsetAsync(context->raster_pairs, 0, sizeof(unsigned long long), context->stream));
+    GSN_HIP(hipStreamSynchronize(context->stream));
+    memcpy(pairs, host, sizeof(unsigned long long));
     return GSN_OK;
 }

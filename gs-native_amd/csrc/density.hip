@@ -373,7 +373,7 @@ extern "C" int gsn_density_accumulate(gsn_context* context, const float* statist
     if (!statistics || !absolute_statistics || !gradient_sums || !absolute_gradient_sums || !observation_counts) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_density_accumulate: null buffer");
     }
-    StageScope scope(context, kStageDensity);
+    StageScope scope(context, kStageAccumulate);
     density_accumulate_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(statistics, absolute_statistics, count,
         gradient_sums, absolute_gradient_sums, observation_counts);
     return check_launch("density_accumulate_kernel");

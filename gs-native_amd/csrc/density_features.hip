@@ -171,7 +171,7 @@ extern "C" int gsn_density_features_accumulate(gsn_context* context, const float
     if (!statistics || !absolute_statistics || !depths || !conic_opacities || !state) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_density_features_accumulate: null buffer");
     }
-    StageScope scope(context, kStageDensity);
+    StageScope scope(context, kStageAccumulate);
     feature_accumulate_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(statistics, absolute_statistics,
         depths, reinterpret_cast<const float4*>(conic_opacities), count, state, stride);
     return check_launch("feature_accumulate_kernel");

@@ -57,6 +57,7 @@ enum Stage {
     kStageAdam,
     kStageDensity,
     kStageKnn,
+    kStageAccumulate,
     kStageCount
 };
 
@@ -89,6 +90,10 @@ struct gsn_context {
     int* event_stage = nullptr;
     int event_capacity = 0;  // number of pairs
     int event_used = 0;
+    // Raster pair counting (diagnostic): while enabled, the raster forward adds the number of
+    // (pixel, splat) pairs it evaluates to *raster_pairs (device memory owned by the context).
+    bool count_raster_pairs = false;
+    unsigned long long* raster_pairs = nullptr;
 };
 
 namespace gsn {

@@ -9,11 +9,14 @@ namespace {
 
 // Store = true additionally writes, per pixel, the final transmittance and the tile-relative
 // index + 1 of the last splat blended (0 when none): the state a back-to-front backward needs.
-template<bool Store>
+// Count = true adds the number of (pixel, splat) pairs evaluated (pixel not yet done, splat's
+// band kept by the mask) to *pair_count: the operation count of the VALU roofline.
+template<bool Store, bool Count>
 __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restrict__ ids,
     const int64_t* __restrict__ tile_offsets, const float2* __restrict__ means, const float4* __restrict__ conics,
     const float* __restrict__ colors, Camera cam, float* __restrict__ image, float bg_r, float bg_g, float bg_b,
-    int tile_count, float* __restrict__ final_transmittance, int* __restrict__ last_contributor) {
+    int tile_count, float* __restrict__ final_transmittance, int* __restrict__ last_contributor,
+    unsigned long long* __restrict__ pair_count) {
     const int tile = xcd_remap(blockIdx.x, tile_count);
     const int lane = threadIdx.x;
     const int x0 = (tile % cam.tile_count_x) * kTileSize;
@@ -23,6 +26,7 @@ __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restric
     const float fx = static_cast<float>(px);
     float fy[4], T[4], cr[4], cg[4], cb[4];
     int last[4] = {0, 0, 0, 0};
+    unsigned evaluated = 0;
     bool done[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -81,6 +85,9 @@ __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restric
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 if (smask & (1 << b)) {
+                    if (Count) {
+                        evaluated += done[b] ? 0u : 1u;
+                    }
                     float dy = sy - fy[b];
                     float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbv * dx * dy;
                     float alpha = fminf(kAlphaMax, op * __expf(power));
@@ -118,6 +125,16 @@ __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restric
             }
         }
     }
+    if (Count) {
+        unsigned long long total = evaluated;
+#pragma unroll
+        for (int offset = 32; offset > 0; offset >>= 1) {
+            total += __shfl_xor(total, offset);
+        }
+        if (lane == 0) {
+            atomicAdd(pair_count, total);
+        }
+    }
 }
 
 } // namespace
@@ -139,18 +156,30 @@ int render_tiles(gsn_context* context, const gsn_camera* camera, const int* tile
     }
     Camera cam;
     memcpy(&cam, camera, sizeof(cam));
+    unsigned long long* pair_count = context->count_raster_pairs ? context->raster_pairs : nullptr;
     StageScope scope(context, kStageRasterForward);
-    if (final_transmittance) {
-        raster_forward_kernel<true><<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(tile_gaussian_ids,
-            tile_offsets, reinterpret_cast<const float2*>(means), reinterpret_cast<const float4*>(conic_opacities),
+    const unsigned grid = static_cast<unsigned>(tiles);
+    const float2* m = reinterpret_cast<const float2*>(means);
+    const float4* q = reinterpret_cast<const float4*>(conic_opacities);
+    if (final_transmittance && pair_count) {
+        raster_forward_kernel<true, true><<<grid, 64, 0, context->stream>>>(tile_gaussian_ids, tile_offsets, m, q,
             colors, cam, image, background[0], background[1], background[2], static_cast<int>(tiles),
-            final_transmittance, last_contributor);
+            final_transmittance, last_contributor, pair_count);
+    }
+    else if (final_transmittance) {
+        raster_forward_kernel<true, false><<<grid, 64, 0, context->stream>>>(tile_gaussian_ids, tile_offsets, m, q,
+            colors, cam, image, background[0], background[1], background[2], static_cast<int>(tiles),
+            final_transmittance, last_contributor, nullptr);
+    }
+    else if (pair_count) {
+        raster_forward_kernel<false, true><<<grid, 64, 0, context->stream>>>(tile_gaussian_ids, tile_offsets, m, q,
+            colors, cam, image, background[0], background[1], background[2], static_cast<int>(tiles), nullptr,
+            nullptr, pair_count);
     }
     else {
-        raster_forward_kernel<false><<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(tile_gaussian_ids,
-            tile_offsets, reinterpret_cast<const float2*>(means), reinterpret_cast<const float4*>(conic_opacities),
+        raster_forward_kernel<false, false><<<grid, 64, 0, context->stream>>>(tile_gaussian_ids, tile_offsets, m, q,
             colors, cam, image, background[0], background[1], background[2], static_cast<int>(tiles), nullptr,
-            nullptr);
+            nullptr, nullptr);
     }
     return check_launch("raster_forward_kernel");
 }

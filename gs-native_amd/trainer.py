@@ -18,13 +18,13 @@ import sys
 import numpy as np
 
 if __package__:
-    from . import DeviceBuffer, Projection, adam_corrections, call, gaussian_gradients_adam, image_gradients, render
-    from . import _p
+    from . import DeviceBuffer, Projection, adam_corrections, call, gaussian_gradients, gaussian_gradients_adam
+    from . import image_gradients, render, _p
 else:  # loaded by path
     _pkg = sys.modules["gs_native_amd"]
     DeviceBuffer, Projection, adam_corrections, call = _pkg.DeviceBuffer, _pkg.Projection, _pkg.adam_corrections, _pkg.call
-    gaussian_gradients_adam, image_gradients, render, _p = (_pkg.gaussian_gradients_adam, _pkg.image_gradients,
-                                                             _pkg.render, _pkg._p)
+    gaussian_gradients, gaussian_gradients_adam = _pkg.gaussian_gradients, _pkg.gaussian_gradients_adam
+    image_gradients, render, _p = _pkg.image_gradients, _pkg.render, _pkg._p
 
 
 FEATURES = 16
@@ -60,7 +60,10 @@ class OutcomeLog:
 class Trainer:
     """policy: None for the gradient-threshold rule, or a ProposalNetwork whose logits choose the
     densification at every density step. outcome_log: an OutcomeLog to fill with (features,
-    outcome label) samples; it labels each density step at the next one."""
+    outcome label) samples; it labels each density step at the next one. fused (attribute): True
+    runs the gradient pass with Adam fused in (no gradient buffer); False writes the [58][N]
+    gradients and runs the separate row-major Adam kernel, with bit-identical results (used to
+    time Adam on its own)."""
 
     def __init__(self, context, params: np.ndarray, cameras: list, targets: list, scene_extent: float,
                  background=(0.0, 0.0, 0.0), max_count: int | None = None, completed: int = 0, seed: int = 0,
@@ -101,6 +104,8 @@ class Trainer:
         self.last_camera = -1
         self.history = []
         self.feature_state = None
+        self.fused = True
+        self.gradients = None
         if self.features_enabled:
             self._reset_feature_state()
 
@@ -135,9 +140,17 @@ class Trainer:
         image_gradients(self.context, camera, self.image, self.targets[index], self.image_grad)
         self.adam_step += 1
         c1, c2 = adam_corrections(self.adam_step)
-        gaussian_gradients_adam(self.context, self.projection, self.params, self.sh_degree, self.image,
-                                self.image_grad, self.first, self.second, lr, c1, c2, self.statistics,
-                                self.absolute_statistics)
+        if self.fused:
+            gaussian_gradients_adam(self.context, self.projection, self.params, self.sh_degree, self.image,
+                                    self.image_grad, self.first, self.second, lr, c1, c2, self.statistics,
+                                    self.absolute_statistics)
+        else:
+            if self.gradients is None or self.gradients.shape != (58, self.count):
+                self.gradients = DeviceBuffer((58, self.count), np.float32)
+            gaussian_gradients(self.context, self.projection, self.params, self.sh_degree, self.image,
+                               self.image_grad, self.gradients, self.statistics, self.absolute_statistics)
+            call("gsn_adam_update", self.context.handle, self.count, self.gradients.ptr, self.params.ptr,
+                 self.first.ptr, self.second.ptr, lr, c1, c2)
         self.completed = iteration
         records = self.projection.record_count
         if self.completed < 15000:

@@ -85,14 +85,22 @@ int gsn_device_memory_info(size_t* free_bytes, size_t* total_bytes);
  * launches of each stage, and clears the record. Stage order:
  *   0 project, 1 scan, 2 depth sort, 3 record emit, 4 tile sort, 5 tile ranges,
  *   6 raster forward, 7 image gradients, 8 raster backward, 9 per-Gaussian gradients (+Adam),
- *   10 Adam, 11 density control, 12 KNN. */
-#define GSN_STAGE_COUNT 13
+ *   10 Adam, 11 density control (count, write, history remap, opacity cap), 12 KNN,
+ *   13 statistics accumulation (per-view density and feature accumulators). */
+#define GSN_STAGE_COUNT 14
 int gsn_context_set_timing(gsn_context* context, int enabled);
 /* Times only the stages whose bit (1 << stage) is set in stage_mask (0 disables timing). Each
  * event record costs the stream a short idle gap, so a throughput measurement times only the
  * stages it needs. No counterpart in the reference. */
 int gsn_context_set_timing_mask(gsn_context* context, unsigned stage_mask);
 int gsn_context_stage_times(gsn_context* context, double* milliseconds, int* launches, int stages);
+/* Raster pair counting (diagnostic, no counterpart in the reference). While enabled, every raster
+ * forward adds the number of (pixel, splat) pairs it evaluates (pixel not yet done, splat's band
+ * kept by the conservative band mask) to a device counter of the context; the backward replays the
+ * same pairs. gsn_context_raster_pairs synchronises, returns the count since the last read and
+ * clears it. This is the operation count of the rasteriser's VALU roofline. */
+int gsn_context_count_raster_pairs(gsn_context* context, int enabled);
+int gsn_context_raster_pairs(gsn_context* context, unsigned long long* pairs);
 
 /* Diagnostic: the wave64 reduction used by the raster backward, on one wave.
  * input float[64][12] -> output float[64][3]; lane l receives the sums over all lanes of

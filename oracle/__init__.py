@@ -46,6 +46,10 @@ def _load() -> ctypes.CDLL:
     F = ctypes.c_float
     CAM = ctypes.POINTER(Camera)
     signatures = {
+        "oracle_set_threads": ([I], None),
+        "oracle_rotation_f32": ([F, F, F, P], None),
+        "oracle_rotation_f64": ([ctypes.c_double, ctypes.c_double, ctypes.c_double, P], None),
+        "oracle_sh_basis_f64": ([ctypes.c_double, ctypes.c_double, ctypes.c_double, P], None),
         "oracle_expf": ([F], F),
         "oracle_logf": ([F], F),
         "oracle_sincosf": ([F, P, P], None),
@@ -86,6 +90,29 @@ def _ptr(array):
 
 def _f32(a):
     return np.ascontiguousarray(a, np.float32)
+
+
+def set_threads(threads: int) -> None:
+    """Threads of the oracle's parallel loops (1 = sequential, the default)."""
+    lib.oracle_set_threads(int(threads))
+
+
+def rotation_f32(rotation) -> np.ndarray:
+    out = np.zeros(9, np.float32)
+    lib.oracle_rotation_f32(*(float(v) for v in rotation), _ptr(out))
+    return out.reshape(3, 3)
+
+
+def rotation_f64(rotation) -> np.ndarray:
+    out = np.zeros(9)
+    lib.oracle_rotation_f64(*(float(v) for v in rotation), _ptr(out))
+    return out.reshape(3, 3)
+
+
+def sh_basis_f64(direction) -> np.ndarray:
+    out = np.zeros(16)
+    lib.oracle_sh_basis_f64(*(float(v) for v in direction), _ptr(out))
+    return out
 
 
 def expf(x: float) -> float:

@@ -11,6 +11,16 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Threads of the parallel loops (OpenMP). 1 by default: the loops then run in the sequential order
+ * the tests were written against. Only the CPU timing of a full iteration raises it; every loop
+ * that is split keeps each output's summation order, except the splat backward, whose per-thread
+ * partial sums are added in thread order (deterministic for a given thread count). */
+static int g_threads = 1;
+
+void oracle_set_threads(int threads) {
+    g_threads = threads < 1 ? 1 : threads;
+}
+
 #define PARAMS 58
 #define SH_INDEX 10
 #define OPACITY_INDEX 9
@@ -511,6 +521,19 @@ static void sh_basis_f64(double x, double y, double z, double basis[16]) {
     basis[15] = -0.5900435899266435 * x * (x * x - 3 * y * y);
 }
 
+/* Exported for the pinning tests (renderer.cu:35-66 and :68-106 as the oracle restates them). */
+void oracle_rotation_f32(float rx, float ry, float rz, float* R) {
+    rotation_f32(rx, ry, rz, R);
+}
+
+void oracle_rotation_f64(double rx, double ry, double rz, double* R) {
+    rotation_f64(rx, ry, rz, R);
+}
+
+void oracle_sh_basis_f64(double x, double y, double z, double* basis) {
+    sh_basis_f64(x, y, z, basis);
+}
+
 void oracle_project_f64(const float* params, int count, const oracle_camera* cam, int sh_degree, double* means,
     double* conics, double* colors, double* depths, int* tile_rects, uint8_t* visible) {
     sh_degree = sh_degree < 0 ? 0 : (sh_degree > 3 ? 3 : sh_degree);
@@ -522,6 +545,7 @@ void oracle_project_f64(const float* params, int count, const oracle_camera* cam
     for (int k = 0; k < 3; ++k) {
         tc[k] = cam->t_c_w[k];
     }
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int i = 0; i < count; ++i) {
         visible[i] = 0;
         double p0 = params[0 * n + i], p1 = params[1 * n + i], p2 = params[2 * n + i];
@@ -687,6 +711,7 @@ int oracle_render_f64(const oracle_camera* cam, int count, const double* means, 
     int64_t* offsets = NULL;
     int* ids = tile_buckets(cam, order, ordered, rects, &offsets);
     int ambiguous_count = 0;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 4) reduction(+ : ambiguous_count)
     for (int y = 0; y < cam->height; ++y) {
         for (int x = 0; x < cam->width; ++x) {
             int tile = (y / 16) * cam->tile_count_x + x / 16;
@@ -757,6 +782,7 @@ static void gaussian_window(double w[11]) {
 /* Separable 11-tap filter of one channel with zero padding (sepFilter2D, BORDER_CONSTANT). */
 static void filter_f64(const double* input, double* output, int width, int height, const double w[11]) {
     double* temp = (double*)malloc(sizeof(double) * (size_t)width * height);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int y = 0; y < height; ++y) {
         for (int x = 0; x < width; ++x) {
             double s = 0.0;
@@ -769,6 +795,7 @@ static void filter_f64(const double* input, double* output, int width, int heigh
             temp[(int64_t)y * width + x] = s;
         }
     }
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int y = 0; y < height; ++y) {
         for (int x = 0; x < width; ++x) {
             double s = 0.0;
@@ -842,21 +869,27 @@ void oracle_image_gradients_f64(const double* image, const uint8_t* target, int 
     }
     double* prod = (double*)malloc(sizeof(double) * pixels);
     double normalization = 1.0 / (3.0 * (double)width * (double)height);
+    const int64_t count = (int64_t)pixels;
     for (int ch = 0; ch < 3; ++ch) {
-        for (size_t p = 0; p < pixels; ++p) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int64_t p = 0; p < count; ++p) {
             a[p] = image[p * 3 + ch];
             b[p] = target[p * 3 + ch] / 255.0;
         }
         filter_f64(a, m[0], width, height, w);
         filter_f64(b, m[1], width, height, w);
-        for (size_t p = 0; p < pixels; ++p) prod[p] = a[p] * a[p];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int64_t p = 0; p < count; ++p) prod[p] = a[p] * a[p];
         filter_f64(prod, m[2], width, height, w);
-        for (size_t p = 0; p < pixels; ++p) prod[p] = b[p] * b[p];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int64_t p = 0; p < count; ++p) prod[p] = b[p] * b[p];
         filter_f64(prod, m[3], width, height, w);
-        for (size_t p = 0; p < pixels; ++p) prod[p] = a[p] * b[p];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int64_t p = 0; p < count; ++p) prod[p] = a[p] * b[p];
         filter_f64(prod, m[4], width, height, w);
         /* solver.cu:112-123: derivative coefficients per window position. */
-        for (size_t p = 0; p < pixels; ++p) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int64_t p = 0; p < count; ++p) {
             double mu_a = m[0][p], mu_b = m[1][p];
             double inverse_d = 1.0 / (mu_a * mu_a + mu_b * mu_b + 0.0001);
             double inverse_e = 1.0 / (m[2][p] - mu_a * mu_a + m[3][p] - mu_b * mu_b + 0.0009);
@@ -873,7 +906,8 @@ void oracle_image_gradients_f64(const double* image, const uint8_t* target, int 
         for (int k = 0; k < 3; ++k) {
             filter_f64(k3[k], f3[k], width, height, w);
         }
-        for (size_t p = 0; p < pixels; ++p) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int64_t p = 0; p < count; ++p) {
             float render = (float)image[p * 3 + ch];
             float reference = (float)target[p * 3 + ch] * (1.0f / 255.0f);
             float residual = render - reference;
@@ -894,20 +928,13 @@ void oracle_image_gradients_f64(const double* image, const uint8_t* target, int 
 
 /* ---- splat backward (solver.cu:325-466), FP64 ------------------------------------------- */
 
-void oracle_backward_f64(const oracle_camera* cam, int count, const double* means, const double* conics,
-    const double* colors, const double* depths, const int* rects, const uint8_t* visible, const double* image,
-    const double* image_gradients, const uint8_t* pixel_flags, double* screen, double* magnitude,
-    uint8_t* tainted) {
+/* Pixel rows [y_begin, y_end) of the replay, accumulated into screen / magnitude / tainted. */
+static void backward_rows(const oracle_camera* cam, int count, const double* means, const double* conics,
+    const double* colors, const int64_t* offsets, const int* ids, const double* image, const double* image_gradients,
+    const uint8_t* pixel_flags, int y_begin, int y_end, double* screen, double* magnitude, uint8_t* tainted) {
     const int64_t n = count;
-    memset(screen, 0, sizeof(double) * 11 * (size_t)count);
-    memset(magnitude, 0, sizeof(double) * 11 * (size_t)count);
-    memset(tainted, 0, (size_t)count);
-    int ordered = 0;
-    int* order = depth_order(count, depths, visible, &ordered);
-    int64_t* offsets = NULL;
-    int* ids = tile_buckets(cam, order, ordered, rects, &offsets);
     int* touched = (int*)malloc(sizeof(int) * 4096);
-    for (int y = 0; y < cam->height; ++y) {
+    for (int y = y_begin; y < y_end; ++y) {
         for (int x = 0; x < cam->width; ++x) {
             int tile = (y / 16) * cam->tile_count_x + x / 16;
             const double* final_color = image + ((int64_t)y * cam->width + x) * 3;
@@ -997,6 +1024,58 @@ void oracle_backward_f64(const oracle_camera* cam, int count, const double* mean
         }
     }
     free(touched);
+}
+
+void oracle_backward_f64(const oracle_camera* cam, int count, const double* means, const double* conics,
+    const double* colors, const double* depths, const int* rects, const uint8_t* visible, const double* image,
+    const double* image_gradients, const uint8_t* pixel_flags, double* screen, double* magnitude,
+    uint8_t* tainted) {
+    const size_t n = (size_t)count;
+    memset(screen, 0, sizeof(double) * 11 * n);
+    memset(magnitude, 0, sizeof(double) * 11 * n);
+    memset(tainted, 0, n);
+    int ordered = 0;
+    int* order = depth_order(count, depths, visible, &ordered);
+    int64_t* offsets = NULL;
+    int* ids = tile_buckets(cam, order, ordered, rects, &offsets);
+    int threads = g_threads < cam->height ? g_threads : (cam->height > 0 ? cam->height : 1);
+    if (threads <= 1) {
+        backward_rows(cam, count, means, conics, colors, offsets, ids, image, image_gradients, pixel_flags, 0,
+            cam->height, screen, magnitude, tainted);
+    }
+    else {
+        /* Thread t replays a contiguous band of rows into its own buffers (thread 0 into the
+         * outputs); the bands are then added in thread order. */
+        double* partial_screen = (double*)calloc(11 * n * (size_t)(threads - 1), sizeof(double));
+        double* partial_magnitude = (double*)calloc(11 * n * (size_t)(threads - 1), sizeof(double));
+        uint8_t* partial_tainted = (uint8_t*)calloc(n * (size_t)(threads - 1), 1);
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+        for (int t = 0; t < threads; ++t) {
+            int y_begin = (int)((int64_t)cam->height * t / threads);
+            int y_end = (int)((int64_t)cam->height * (t + 1) / threads);
+            double* s_out = t == 0 ? screen : partial_screen + 11 * n * (size_t)(t - 1);
+            double* m_out = t == 0 ? magnitude : partial_magnitude + 11 * n * (size_t)(t - 1);
+            uint8_t* f_out = t == 0 ? tainted : partial_tainted + n * (size_t)(t - 1);
+            backward_rows(cam, count, means, conics, colors, offsets, ids, image, image_gradients, pixel_flags,
+                y_begin, y_end, s_out, m_out, f_out);
+        }
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t i = 0; i < (int64_t)(11 * n); ++i) {
+            for (int t = 1; t < threads; ++t) {
+                screen[i] += partial_screen[11 * n * (size_t)(t - 1) + (size_t)i];
+                magnitude[i] += partial_magnitude[11 * n * (size_t)(t - 1) + (size_t)i];
+            }
+        }
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            for (int t = 1; t < threads; ++t) {
+                tainted[i] |= partial_tainted[n * (size_t)(t - 1) + (size_t)i];
+            }
+        }
+        free(partial_screen);
+        free(partial_magnitude);
+        free(partial_tainted);
+    }
     free(ids);
     free(offsets);
     free(order);
@@ -1013,6 +1092,7 @@ void oracle_map_f64(const float* params, int count, const oracle_camera* cam, in
     for (int k = 0; k < 9; ++k) Rc[k] = cam->R_c_w[k];
     for (int k = 0; k < 3; ++k) tc[k] = cam->t_c_w[k];
     memset(gradients, 0, sizeof(double) * PARAMS * (size_t)count);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int id = 0; id < count; ++id) {
         double in[11];
         for (int k = 0; k < 11; ++k) in[k] = screen[k * n + id];
@@ -1175,6 +1255,7 @@ void oracle_adam_f32(int count, const float* gradients, float* params, float* fi
     float position_learning_rate, float first_correction, float second_correction) {
     for (int row = 0; row < PARAMS; ++row) {
         float lr = learning_rate_for_row(row, position_learning_rate);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
         for (int id = 0; id < count; ++id) {
             int64_t index = (int64_t)row * count + id;
             float g = gradients[index];

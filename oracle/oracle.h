@@ -33,6 +33,16 @@ typedef struct oracle_camera {
     int tile_offset;
 } oracle_camera;
 
+/* Threads of the parallel loops (OpenMP) of the FP64 iteration functions (projection, render,
+ * image gradients, splat backward, chain rule) and of Adam. Default 1: sequential order. */
+void oracle_set_threads(int threads);
+
+/* Axis-angle to row-major rotation (renderer.cu:35-66) in FP32 (the device's operation order)
+ * and FP64, and the 16 real SH basis values of a unit direction (renderer.cu:68-106) in FP64. */
+void oracle_rotation_f32(float rx, float ry, float rz, float* R);
+void oracle_rotation_f64(double rx, double ry, double rz, double* R);
+void oracle_sh_basis_f64(double x, double y, double z, double* basis);
+
 /* Project-written FP32 transcendentals (identical algorithm to the device code). */
 float oracle_expf(float x);
 float oracle_logf(float x);
