@@ -104,6 +104,7 @@ _PROTOTYPES = {
     "gsn_render_tiles": [_P, _CAM, _P, _P, _P, _P, _P, ctypes.POINTER(_F), _P],
     "gsn_image_gradients": [_P, _I, _I, _P, _P, _P],
     "gsn_image_metrics": [_P, _I, _I, _P, _P, ctypes.POINTER(ctypes.c_double)],
+    "gsn_image_metrics_static": [_P, _I, _I, _P, _P, _P],
     "gsn_gaussian_gradients": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                     _F, _F, _F, _P, _P],

@@ -85,6 +85,9 @@ int gsn_context_create(gsn_context** context) {
             "hipDeviceGetAttribute");
     }
     if (status == GSN_OK) {
+        status = gsn::resolve_static_scratch(created->device, &created->static_scratch);
+    }
+    if (status == GSN_OK) {
         status = gsn::check_hip(hipStreamCreate(&created->stream), "hipStreamCreate");
     }
     if (status == GSN_OK) {

@@ -94,6 +94,9 @@ struct gsn_context {
     // (pixel, splat) pairs it evaluates to *raster_pairs (device memory owned by the context).
     bool count_raster_pairs = false;
     unsigned long long* raster_pairs = nullptr;
+    // Device address of the allocation-free metrics' file-scope array on this context's device
+    // (resolved by gsn_context_create).
+    void* static_scratch = nullptr;
 };
 
 namespace gsn {
@@ -119,6 +122,15 @@ int check_hip(hipError_t error, const char* what);
 int check_launch(const char* kernel);
 // Returns a device buffer of at least `bytes` for `slot`, growing it if needed (no copy).
 void* scratch(gsn_context* context, ScratchSlot slot, size_t bytes);
+
+// Image metrics (ssim.hip): partial sums per 32x16 block and channel into block_sums
+// (image_metric_blocks(width, height) x 3 doubles), then a fixed-order finish into out[4] (device).
+int64_t image_metric_blocks(int width, int height);
+int launch_image_metrics(gsn_context* context, int width, int height, const float* image, const uint8_t* target,
+    double* block_sums, double* out);
+// Device address of the allocation-free metrics' array on `device` (static_metrics.hip); the
+// device must be current.
+int resolve_static_scratch(int device, void** address);
 
 // Device-wide exclusive prefix scan of int / int4-like arrays (scan.hip).
 struct Int4 {

@@ -366,33 +366,77 @@ extern "C" int gsn_image_gradients(gsn_context* context, int width, int height, 
     return check_launch("image_gradients_kernel");
 }
 
+namespace gsn {
+namespace {
+
+// Fixed-order finish of the per-block metric sums: lane l adds blocks l, l + 64, ... in order,
+// then the 64 lane sums are combined by a butterfly of fixed shape, so every call with the same
+// inputs returns the same bits. out = {0.8 L1 + 0.2 (1 - SSIM), L1, SSIM, MSE}.
+__global__ __launch_bounds__(64) void image_metrics_finish_kernel(const double* __restrict__ block_sums,
+    int64_t blocks, double values, double* __restrict__ out) {
+    const int lane = threadIdx.x;
+    double sums[3] = {0.0, 0.0, 0.0};
+    for (int64_t b = lane; b < blocks; b += 64) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sums[k] += block_sums[b * 3 + k];
+        }
+    }
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sums[k] += __shfl_xor(sums[k], offset);
+        }
+    }
+    if (lane == 0) {
+        const double l1 = sums[0] / values;
+        const double ssim = sums[1] / values;
+        out[0] = 0.8 * l1 + 0.2 * (1.0 - ssim);
+        out[1] = l1;
+        out[2] = ssim;
+        out[3] = sums[2] / values;
+    }
+}
+
+} // namespace
+
+int64_t image_metric_blocks(int width, int height) {
+    return static_cast<int64_t>(grid_for(width, kTw)) * grid_for(height, kTh) * 3;
+}
+
+int launch_image_metrics(gsn_context* context, int width, int height, const float* image, const uint8_t* target,
+    double* block_sums, double* out) {
+    dim3 grid(grid_for(width, kTw), grid_for(height, kTh), 3);
+    image_metrics_kernel<<<grid, kThreads, 0, context->stream>>>(image, target, block_sums, width, height, window());
+    GSN_TRY(check_launch("image_metrics_kernel"));
+    const double values = 3.0 * static_cast<double>(width) * height;
+    image_metrics_finish_kernel<<<1, 64, 0, context->stream>>>(block_sums, image_metric_blocks(width, height), values,
+        out);
+    return check_launch("image_metrics_finish_kernel");
+}
+
+} // namespace gsn
+
 extern "C" int gsn_image_metrics(gsn_context* context, int width, int height, const float* image,
     const uint8_t* target, double* out) {
     using namespace gsn;
     if (!context || width <= 0 || height <= 0 || !image || !target || !out) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_image_metrics: invalid arguments");
     }
-    dim3 grid(grid_for(width, kTw), grid_for(height, kTh), 3);
-    size_t blocks = static_cast<size_t>(grid.x) * grid.y * grid.z;
-    double* sums = static_cast<double*>(scratch(context, kSlotReduce, blocks * 3 * sizeof(double)));
+    const size_t blocks = static_cast<size_t>(image_metric_blocks(width, height));
+    // Block sums followed by the four results.
+    double* sums = static_cast<double*>(scratch(context, kSlotReduce, (blocks * 3 + 4) * sizeof(double)));
     if (!sums) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
-    image_metrics_kernel<<<grid, kThreads, 0, context->stream>>>(image, target, sums, width, height, window());
-    GSN_TRY(check_launch("image_metrics_kernel"));
-    std::vector<double> host(blocks * 3);
-    GSN_HIP(hipMemcpyAsync(host.data(), sums, host.size() * sizeof(double), hipMemcpyDeviceToHost, context->stream));
+    double* results = sums + blocks * 3;
+    GSN_TRY(launch_image_metrics(context, width, height, image, target, sums, results));
+    double* host = static_cast<double*>(context->pinned);
+    GSN_HIP(hipMemcpyAsync(host, results, 4 * sizeof(double), hipMemcpyDeviceToHost, context->stream));
     GSN_HIP(hipStreamSynchronize(context->stream));
-    double l1 = 0.0, ssim = 0.0, squared = 0.0;
-    for (size_t b = 0; b < blocks; ++b) {
-        l1 += host[b * 3 + 0];
-        ssim += host[b * 3 + 1];
-        squared += host[b * 3 + 2];
+    for (int k = 0; k < 4; ++k) {
+        out[k] = host[k];
     }
-    double values = 3.0 * static_cast<double>(width) * height;
-    out[1] = l1 / values;
-    out[2] = ssim / values;
-    out[0] = 0.8 * out[1] + 0.2 * (1.0 - out[2]);
-    out[3] = squared / values;
     return GSN_OK;
 }

@@ -171,6 +171,17 @@ int gsn_image_gradients(gsn_context* context, int width, int height, const float
 int gsn_image_metrics(gsn_context* context, int width, int height, const float* image,
     const uint8_t* target, double* out);
 
+/* Allocation-free gsn_image_metrics: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, evaluation loops that must not call
+ * hipMalloc). Same kernels and results (bit for bit) as gsn_image_metrics, but the per-block
+ * partial sums live in a file-scope device array of 3 MiB per device (131,072 blocks of 32x16
+ * pixels per channel: images up to about 22 million pixels; larger ones return
+ * GSN_ERROR_CAPACITY), and the four results are written to device_out (4 doubles of device
+ * memory) without synchronising the stream. Calls that overlap on one device share that array
+ * and race; calls on one context's stream are ordered and safe. */
+int gsn_image_metrics_static(gsn_context* context, int width, int height, const float* image,
+    const uint8_t* target, double* device_out);
+
 /* Splat backward (recomputes transmittance front to back from the final colour), projection
  * statistics and the screen-to-parameter chain rule.
  * Replaces ComputeSplatGradientsKernel, CollectProjectionStatisticsKernel and
