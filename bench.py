@@ -66,6 +66,15 @@ def load_module(name: str, path: pathlib.Path):
     return module
 
 
+def load_module_into_package(name: str):
+    """A module of the package, registered as gs_native_amd.<name> (modules refer to each other
+    through sys.modules when loaded by path)."""
+    full = f"gs_native_amd.{name}"
+    if full not in sys.modules:
+        sys.modules[full] = load_module(full, REPO / "gs-native_amd" / f"{name}.py")
+    return sys.modules[full]
+
+
 def log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
@@ -113,39 +122,6 @@ def profiled_value(key: str, stage: str):
     return None, None
 
 
-def build_workload(gsn, context, gaussians, width, height, cameras, seed=0, scene="blobs", camera_radius=5.0,
-                   hemisphere=False):
-    scenes = load_module("gs_native_amd.scenes", REPO / "gs-native_amd" / "scenes.py")
-    if scene == "shell":
-        truth = scenes.shell_plane_scene(gaussians, seed=seed, sh_degree=3)
-    else:
-        truth = scenes.blob_scene(gaussians, seed=seed, sh_degree=3)
-    _, mean_dist2 = gsn.knn3(context, truth[0:3].T.copy())
-    truth[3:6] = scenes.scales_from_mean_dist2(mean_dist2)[None, :]
-    params = scenes.perturb(truth, seed + 1, 0.01)
-    focal = scenes.fov_focal(width, 60.0)
-    poses = scenes.sphere_cameras(cameras, camera_radius, seed + 2, hemisphere=hemisphere)
-    cams = [gsn.Camera.make(width, height, focal, focal, 0.5 * (width - 1), 0.5 * (height - 1), R, t) for R, t in poses]
-    centres = np.stack([-R.T @ t for R, t in poses])
-    extent = 1.1 * float(np.linalg.norm(centres - centres.mean(0), axis=1).max())
-    device_truth = gsn.DeviceBuffer.from_numpy(truth)
-    image = gsn.DeviceBuffer((height, width, 3), np.float32)
-    targets, visible = [], []
-    projection = None
-    for cam in cams:
-        projection = gsn.render(context, device_truth, gaussians, cam, (0.0, 0.0, 0.0), image, 3, projection)
-        target = np.clip(np.rint(image.numpy() * 255.0), 0, 255).astype(np.uint8)
-        targets.append(gsn.DeviceBuffer.from_numpy(target))
-    del device_truth
-    device_params = gsn.DeviceBuffer.from_numpy(params)
-    for cam in cams:
-        projection = gsn.render(context, device_params, gaussians, cam, (0.0, 0.0, 0.0), image, 3, projection)
-        counts = np.diff(projection.record_offsets.numpy())
-        visible.append(int((counts > 0).sum()))
-    del device_params
-    return params, cams, targets, extent, visible
-
-
 def host_cpu() -> dict:
     model = None
     try:
@@ -170,7 +146,7 @@ def cpu_baseline(gsn, params: np.ndarray, camera, target: np.ndarray, runs: int)
     sys.path.insert(0, str(REPO))
     import oracle
 
-    scenes = load_module("gs_native_amd.scenes", REPO / "gs-native_amd" / "scenes.py")
+    scenes = load_module_into_package("scenes")
     cpu = host_cpu()
     threads = max(1, min(cpu["nproc"] or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
 
@@ -282,11 +258,13 @@ def main() -> int:
         print("bench.py: --steps must be >= 1, --warmup and --stage-steps >= 0", file=sys.stderr)
         return 2
     gsn = load_package()
-    trainer_module = load_module("gs_native_amd.trainer", REPO / "gs-native_amd" / "trainer.py")
+    load_module_into_package("scenes")
+    workloads = load_module_into_package("workloads")
+    trainer_module = load_module_into_package("trainer")
     context = gsn.Context()
     log(f"device {gsn.device_name()}")
     log(f"building workload: {args.gaussians} Gaussians, {args.width}x{args.height}, {args.cameras} cameras")
-    params, cameras, targets, extent, visible = build_workload(
+    params, cameras, targets, extent, visible = workloads.build_workload(
         gsn, context, args.gaussians, args.width, args.height, args.cameras, scene=args.scene,
         camera_radius=args.camera_radius, hemisphere=args.hemisphere)
     log(f"workload ready: mean visible {np.mean(visible):.0f}")

@@ -105,6 +105,7 @@ _PROTOTYPES = {
     "gsn_image_gradients": [_P, _I, _I, _P, _P, _P],
     "gsn_image_metrics": [_P, _I, _I, _P, _P, ctypes.POINTER(ctypes.c_double)],
     "gsn_image_metrics_static": [_P, _I, _I, _P, _P, _P],
+    "gsn_image_quantized_error": [_P, _I, _I, _P, _P, _P, ctypes.POINTER(ctypes.c_double)],
     "gsn_gaussian_gradients": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                     _F, _F, _F, _P, _P],
@@ -409,6 +410,15 @@ def image_metrics(context: Context, width: int, height: int, image: DeviceBuffer
     mse = out[3]
     psnr = float("inf") if mse <= 0 else 10.0 * np.log10(1.0 / mse)
     return {"loss": out[0], "l1": out[1], "ssim": out[2], "mse": mse, "psnr": psnr}
+
+
+def quantized_error(context: Context, width: int, height: int, image: DeviceBuffer, target: DeviceBuffer,
+                    quantized: DeviceBuffer | None = None) -> tuple[float, float]:
+    """(sum of squared 8-bit differences, PSNR dB) of the clamped, half-to-even quantised image
+    against the target: the held-out score of the reference's render program."""
+    out = (ctypes.c_double * 2)()
+    call("gsn_image_quantized_error", context.handle, width, height, _p(image), _p(target), _p(quantized), out)
+    return out[0], out[1]
 
 
 def gaussian_gradients(context: Context, projection: Projection, params: DeviceBuffer, sh_degree: int,

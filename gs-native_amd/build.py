@@ -94,31 +94,56 @@ def build_cpp(jobs: int = 8) -> pathlib.Path | None:
         objects = list(pool.map(compile_one, sources))
     temporary = CPP_LIBRARY.with_suffix(".so.tmp")
     _run(["g++", "-shared", "-o", str(temporary), *map(str, objects), f"-L{PACKAGE}", "-lgsn_hip",
-          f"-L{ROCM / 'lib'}", "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM / 'lib'}"])
+          f"-L{ROCM / 'lib'}", "-lamdhip64", "-lz", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(temporary, CPP_LIBRARY)
     return CPP_LIBRARY
 
 
-CPP_TEST_SOURCE = REPO / "tests" / "cpp" / "api_test.cpp"
-CPP_TEST_BINARY = REPO / "tests" / "cpp" / "api_test"
+CPP_TESTS = REPO / "tests" / "cpp"
+# api_test drives the C++ classes on a GPU; io_test checks the file formats on the CPU.
+CPP_TEST_PROGRAMS = ("api_test", "io_test")
 
 
-def build_cpp_tests() -> pathlib.Path | None:
-    """The C++ API test program (runs on a GPU box; compiled here)."""
-    if not CPP_TEST_SOURCE.exists() or not CPP_LIBRARY.exists():
+def build_cpp_tests() -> list[pathlib.Path]:
+    """The C++ test programs (compiled here; api_test runs on a GPU box)."""
+    built = []
+    if not CPP_LIBRARY.exists():
+        return built
+    for name in CPP_TEST_PROGRAMS:
+        source = CPP_TESTS / f"{name}.cpp"
+        if not source.exists():
+            continue
+        binary = CPP_TESTS / name
+        temporary = binary.with_suffix(".tmp")
+        _run(["g++", *CXX_FLAGS, str(source), "-o", str(temporary), f"-L{PACKAGE}", "-lgs_native", "-lgsn_hip",
+              f"-L{ROCM / 'lib'}", "-lamdhip64", "-lz", f"-Wl,-rpath,{PACKAGE}",
+              "-Wl,-rpath,$ORIGIN/../../gs-native_amd", f"-Wl,-rpath,{ROCM / 'lib'}"])
+        os.replace(temporary, binary)
+        built.append(binary)
+    return built
+
+
+RENDER_PROGRAM_SOURCE = PACKAGE / "demo" / "render.cpp"
+RENDER_PROGRAM = PACKAGE / "demo" / "gs_native_render"
+
+
+def build_render_program() -> pathlib.Path | None:
+    """The held-out render / scoring program (the reference's demo/render.cpp counterpart)."""
+    if not RENDER_PROGRAM_SOURCE.exists() or not CPP_LIBRARY.exists():
         return None
-    temporary = CPP_TEST_BINARY.with_suffix(".tmp")
-    _run(["g++", *CXX_FLAGS, str(CPP_TEST_SOURCE), "-o", str(temporary), f"-L{PACKAGE}", "-lgs_native", "-lgsn_hip",
-          f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{PACKAGE}", "-Wl,-rpath,$ORIGIN/../../gs-native_amd",
+    temporary = RENDER_PROGRAM.with_suffix(".tmp")
+    _run(["g++", *CXX_FLAGS, str(RENDER_PROGRAM_SOURCE), "-o", str(temporary), f"-L{PACKAGE}", "-lgs_native",
+          "-lgsn_hip", f"-L{ROCM / 'lib'}", "-lamdhip64", "-lz", "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{PACKAGE}",
           f"-Wl,-rpath,{ROCM / 'lib'}"])
-    os.replace(temporary, CPP_TEST_BINARY)
-    return CPP_TEST_BINARY
+    os.replace(temporary, RENDER_PROGRAM)
+    return RENDER_PROGRAM
 
 
 def build_all(jobs: int = 8) -> None:
     build_hip(jobs)
     build_cpp(jobs)
     build_cpp_tests()
+    build_render_program()
 
 
 def clean() -> None:

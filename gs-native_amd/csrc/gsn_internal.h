@@ -39,6 +39,7 @@ enum ScratchSlot {
     kSlotDepthSortedKeys,
     kSlotDepthRects,
     kSlotGaussianOrder,
+    kSlotQuantizedError,
     kSlotCount
 };
 

@@ -1,274 +1,317 @@
-// GaussianSplatting over the gs_native classes (reference src/core/gaussian_splatting.cpp:
-// 72-344). Differences: training images are uploaded once at Initialize and stay resident
-// (the reference copies the target every iteration), and no per-iteration log line is printed
-// unless GS_NATIVE_LOG=info.
+// GaussianSplatting: the driver of the training path (API of reference
+// src/core/gaussian_splatting.h:14-53). The schedule lives in training_schedule.h; this file
+// holds the state that survives between iterations (device images, camera cycle, iteration
+// count) and one function per iteration.
+//
+// Training images are uploaded once at Initialize and stay resident in device memory, so an
+// iteration moves no image data over PCIe. The camera cycle reproduces the reference's order: a
+// std::mt19937 seeded with 0 at the start of every Train call reshuffles the running permutation
+// whenever a pass over the views ends.
 #include "core/gaussian_splatting.h"
 
 #include "core/knn/knn.h"
 #include "device_memory.h"
 #include "io/data_io.h"
 #include "log.h"
+#include "training_schedule.h"
 #include "util/device.h"
 
 #include <algorithm>
 #include <cmath>
 #include <limits>
 #include <numeric>
-#include <random>
 
 namespace gs_native {
 
 namespace {
 
-Vector3f CameraCentre(const Camera& camera) {
-    Vector3f centre = Vector3f::Zero();
-    for (int axis = 0; axis < 3; ++axis) {
-        for (int row = 0; row < 3; ++row) {
-            centre[axis] -= camera.R_c_w_(row, axis) * camera.t_c_w_[row];
+constexpr float kShDc = 0.28209479177387814f;       // Y_0^0: colour = 0.5 + kShDc * dc
+constexpr float kInitialOpacity = 0.1f;
+constexpr float kMinMeanDistance2 = 1e-7f;
+
+// Camera centre c = -R^T t of a world-to-camera pose.
+Vector3f CentreOf(const Camera& camera) {
+    Vector3f c = Vector3f::Zero();
+    for (int row = 0; row < 3; ++row) {
+        for (int axis = 0; axis < 3; ++axis) {
+            c[axis] -= camera.R_c_w_(row, axis) * camera.t_c_w_[row];
         }
     }
-    return centre;
+    return c;
+}
+
+float Distance(const Vector3f& a, const Vector3f& b) {
+    return MakeVector3f(a[0] - b[0], a[1] - b[1], a[2] - b[2]).norm();
+}
+
+// Scene extent: 1.1 x the largest distance of a training camera centre from their mean; when all
+// centres coincide, the largest distance of a point from that mean (at least 1e-6).
+float SceneExtent(const std::vector<Camera>& cameras, const std::vector<Vector3f>& points) {
+    Vector3f mean = Vector3f::Zero();
+    for (const Camera& camera : cameras) {
+        Vector3f c = CentreOf(camera);
+        for (int axis = 0; axis < 3; ++axis) {
+            mean[axis] += c[axis];
+        }
+    }
+    for (int axis = 0; axis < 3; ++axis) {
+        mean[axis] /= static_cast<float>(cameras.size());
+    }
+    float spread = 0.0f;
+    for (const Camera& camera : cameras) {
+        spread = std::max(spread, Distance(CentreOf(camera), mean));
+    }
+    if (spread * 1.1f != 0.0f) {
+        return spread * 1.1f;
+    }
+    float reach = 0.0f;
+    for (const Vector3f& point : points) {
+        reach = std::max(reach, Distance(point, mean));
+    }
+    return std::max(reach, 1e-6f);
+}
+
+// Initial [58][N] parameters: positions from the points, isotropic log-scales from the mean
+// squared 3-NN distance (half its log), identity rotation, opacity 0.1, SH DC from the point
+// colours and no higher bands.
+std::vector<float> InitialParameters(const PointCloud& cloud, const std::vector<float>& mean_distance2) {
+    const size_t n = cloud.positions_.size();
+    std::vector<float> rows(n * kMaxParamCount, 0.0f);
+    auto row = [&](int index) { return rows.data() + static_cast<size_t>(index) * n; };
+    const float opacity_logit = std::log(kInitialOpacity / (1.0f - kInitialOpacity));
+    for (size_t i = 0; i < n; ++i) {
+        const float log_scale = 0.5f * std::log(std::max(mean_distance2[i], kMinMeanDistance2));
+        for (int axis = 0; axis < 3; ++axis) {
+            row(kPositionIndex + axis)[i] = cloud.positions_[i][axis];
+            row(kLogScaleIndex + axis)[i] = log_scale;
+            row(kShIndex + 16 * axis)[i] = (cloud.colors_[i][axis] - 0.5f) / kShDc;
+        }
+        row(kOpacityIndex)[i] = opacity_logit;
+    }
+    return rows;
+}
+
+// [58][N] parameters to per-Gaussian records (axis-angle kept; the PLY writer converts it).
+std::vector<Gaussian> GaussiansFromRows(const std::vector<float>& rows, int count) {
+    std::vector<Gaussian> out(static_cast<size_t>(count));
+    auto at = [&](int index, int id) { return rows[static_cast<size_t>(index) * count + id]; };
+    for (int id = 0; id < count; ++id) {
+        Gaussian& g = out[static_cast<size_t>(id)];
+        for (int axis = 0; axis < 3; ++axis) {
+            g.position_[axis] = at(kPositionIndex + axis, id);
+            g.log_scale_[axis] = at(kLogScaleIndex + axis, id);
+            g.rotation_[axis] = at(kRotationIndex + axis, id);
+            for (int k = 0; k < 16; ++k) {
+                g.sh_(k, axis) = at(kShIndex + 16 * axis + k, id);
+            }
+        }
+        g.opacity_logit_ = at(kOpacityIndex, id);
+    }
+    return out;
+}
+
+bool ImageMatches(const Image8& image, const Camera& camera) {
+    return !image.empty() && image.width == camera.image_size_.width && image.height == camera.image_size_.height &&
+           image.data.size() == static_cast<size_t>(image.width) * image.height * 3;
 }
 
 } // namespace
 
 GaussianSplatting::~GaussianSplatting() {
     DeviceFree(gaussians_.gaussians_);
-    DeviceFree(image_);
-    for (uint8_t* target : targets_) {
-        DeviceFree(target);
-    }
+    ReleaseImages();
 }
 
-bool GaussianSplatting::Initialize(const Scene& scene) {
-    GS_NATIVE_INFO("Initializing scene: training cameras %zu, test cameras %zu, points %zu.",
-        scene.train_cameras_.size(), scene.test_cameras_.size(), scene.point_cloud_.positions_.size());
-    if (gaussians_.gaussians_ || scene.train_cameras_.empty() ||
-        scene.train_cameras_.size() != scene.train_images_.size() || scene.point_cloud_.positions_.empty() ||
-        scene.point_cloud_.positions_.size() != scene.point_cloud_.colors_.size() ||
-        scene.point_cloud_.positions_.size() > static_cast<size_t>(std::numeric_limits<int>::max()) || !DefaultContext()) {
-        return false;
-    }
-    size_t pixels = 0;
-    for (size_t i = 0; i < scene.train_cameras_.size(); ++i) {
-        const Camera& camera = scene.train_cameras_[i];
-        const Image8& image = scene.train_images_[i];
-        if (image.empty() || image.width != camera.image_size_.width || image.height != camera.image_size_.height ||
-            image.data.size() != static_cast<size_t>(image.width) * image.height * 3) {
-            return false;
-        }
-        pixels = std::max(pixels, static_cast<size_t>(image.width) * image.height);
-    }
-    scene_ = scene;
-    Vector3f mean = Vector3f::Zero();
-    for (const Camera& camera : scene_.train_cameras_) {
-        Vector3f centre = CameraCentre(camera);
-        for (int axis = 0; axis < 3; ++axis) {
-            mean[axis] += centre[axis];
-        }
-    }
-    for (int axis = 0; axis < 3; ++axis) {
-        mean[axis] /= static_cast<float>(scene_.train_cameras_.size());
-    }
-    scene_extent_ = 0.0f;
-    for (const Camera& camera : scene_.train_cameras_) {
-        Vector3f centre = CameraCentre(camera);
-        Vector3f d = MakeVector3f(centre[0] - mean[0], centre[1] - mean[1], centre[2] - mean[2]);
-        scene_extent_ = std::max(scene_extent_, d.norm());
-    }
-    scene_extent_ *= 1.1f;
-    if (scene_extent_ == 0.0f) {
-        for (const Vector3f& point : scene_.point_cloud_.positions_) {
-            Vector3f d = MakeVector3f(point[0] - mean[0], point[1] - mean[1], point[2] - mean[2]);
-            scene_extent_ = std::max(scene_extent_, d.norm());
-        }
-        scene_extent_ = std::max(scene_extent_, 1e-6f);
-    }
+void GaussianSplatting::ReleaseImages() {
     DeviceFree(image_);
     image_ = nullptr;
+    image_pixels_ = 0;
     for (uint8_t* target : targets_) {
         DeviceFree(target);
     }
     targets_.clear();
-    pixel_capacity_ = 0;
-    if (!DeviceAllocateCount(&image_, pixels * 3)) {
+}
+
+bool GaussianSplatting::EnsureImageCapacity(size_t pixels) {
+    if (pixels <= image_pixels_) {
+        return true;
+    }
+    float* grown = nullptr;
+    if (!DeviceAllocateCount(&grown, pixels * 3)) {
         return false;
     }
-    for (const Image8& image : scene_.train_images_) {
-        uint8_t* target = nullptr;
-        if (!DeviceAllocateCount(&target, image.data.size()) ||
-            gsn_memcpy_to_device(target, image.data.data(), image.data.size()) != GSN_OK) {
-            DeviceFree(target);
-            return false;
-        }
-        targets_.push_back(target);
-    }
-    pixel_capacity_ = pixels;
-    camera_order_.resize(scene_.train_cameras_.size());
-    std::iota(camera_order_.begin(), camera_order_.end(), 0);
-    camera_cursor_ = static_cast<int>(camera_order_.size());
-    completed_ = 0;
-    sh_degree_ = 0;
-    if (!InitializeGaussians()) {
-        GS_NATIVE_ERROR("Gaussian initialization failed.");
-        return false;
-    }
+    DeviceFree(image_);
+    image_ = grown;
+    image_pixels_ = pixels;
     return true;
 }
 
-bool GaussianSplatting::InitializeGaussians() {
-    const PointCloud& cloud = scene_.point_cloud_;
-    int count = static_cast<int>(cloud.positions_.size());
-    std::vector<float> distances = ComputeMeanDist2Cuda(cloud.positions_);
-    if (distances.size() != cloud.positions_.size()) {
+bool GaussianSplatting::Initialize(const Scene& scene) {
+    const auto& views = scene.train_cameras_;
+    const auto& points = scene.point_cloud_;
+    const bool usable = !gaussians_.gaussians_ && !views.empty() && views.size() == scene.train_images_.size() &&
+                        !points.positions_.empty() && points.positions_.size() == points.colors_.size() &&
+                        points.positions_.size() <= static_cast<size_t>(std::numeric_limits<int>::max()) &&
+                        DefaultContext() != nullptr;
+    if (!usable) {
         return false;
     }
-    std::vector<float> parameters(static_cast<size_t>(count) * kMaxParamCount, 0.0f);
-    for (int id = 0; id < count; ++id) {
-        float scale = 0.5f * std::log(std::max(distances[id], 1e-7f));
-        for (int axis = 0; axis < 3; ++axis) {
-            parameters[static_cast<size_t>(axis) * count + id] = cloud.positions_[id][axis];
-            parameters[static_cast<size_t>(kLogScaleIndex + axis) * count + id] = scale;
-            parameters[static_cast<size_t>(kShIndex + axis * 16) * count + id] =
-                (cloud.colors_[id][axis] - 0.5f) / 0.28209479177387814f;
+    size_t largest = 0;
+    for (size_t v = 0; v < views.size(); ++v) {
+        if (!ImageMatches(scene.train_images_[v], views[v])) {
+            GS_NATIVE_ERROR("training image %zu does not match its camera's %dx%d", v, views[v].image_size_.width,
+                views[v].image_size_.height);
+            return false;
         }
-        parameters[static_cast<size_t>(kOpacityIndex) * count + id] = std::log(0.1f / 0.9f);
+        largest = std::max(largest, scene.train_images_[v].data.size() / 3);
     }
-    float* replacement = nullptr;
-    if (!DeviceAllocateCount(&replacement, parameters.size())) {
+    scene_ = scene;
+    scene_extent_ = SceneExtent(scene_.train_cameras_, scene_.point_cloud_.positions_);
+    ReleaseImages();
+    if (!EnsureImageCapacity(largest)) {
         return false;
     }
-    if (gsn_memcpy_to_device(replacement, parameters.data(), parameters.size() * sizeof(float)) != GSN_OK ||
-        !density_.Initialize(count, count) || !solver_.RemapGaussians(nullptr, count)) {
-        DeviceFree(replacement);
+    for (const Image8& image : scene_.train_images_) {
+        uint8_t* resident = nullptr;
+        bool ok = DeviceAllocateCount(&resident, image.data.size()) &&
+                  gsn_memcpy_to_device(resident, image.data.data(), image.data.size()) == GSN_OK;
+        if (!ok) {
+            DeviceFree(resident);
+            ReleaseImages();
+            return false;
+        }
+        targets_.push_back(resident);
+    }
+    cycle_.resize(views.size());
+    std::iota(cycle_.begin(), cycle_.end(), 0);
+    cycle_position_ = cycle_.size();  // the first Train iteration shuffles
+    completed_ = 0;
+    sh_degree_ = 0;
+
+    const int count = static_cast<int>(points.positions_.size());
+    std::vector<float> mean_distance2 = ComputeMeanDist2Cuda(scene_.point_cloud_.positions_);
+    if (mean_distance2.size() != points.positions_.size()) {
+        GS_NATIVE_ERROR("3-NN initialisation of %d points failed", count);
         return false;
     }
-    DeviceFree(gaussians_.gaussians_);
-    gaussians_.gaussians_ = replacement;
+    std::vector<float> rows = InitialParameters(scene_.point_cloud_, mean_distance2);
+    float* device_rows = nullptr;
+    bool ok = DeviceAllocateCount(&device_rows, rows.size()) &&
+              gsn_memcpy_to_device(device_rows, rows.data(), rows.size() * sizeof(float)) == GSN_OK &&
+              density_.Initialize(count, count) && solver_.RemapGaussians(nullptr, count);
+    if (!ok) {
+        DeviceFree(device_rows);
+        GS_NATIVE_ERROR("cannot place %d initial Gaussians on the device", count);
+        return false;
+    }
+    gaussians_.gaussians_ = device_rows;
     gaussians_.gaussian_count_ = count;
+    GS_NATIVE_INFO("initialised %d Gaussians from %zu views (extent %g)", count, views.size(), scene_extent_);
+    return true;
+}
+
+int GaussianSplatting::NextCamera() {
+    if (cycle_position_ >= cycle_.size()) {
+        std::shuffle(cycle_.begin(), cycle_.end(), shuffler_);
+        cycle_position_ = 0;
+    }
+    return cycle_[cycle_position_];
+}
+
+bool GaussianSplatting::Step() {
+    const int view = NextCamera();
+    const CameraParams camera = scene_.train_cameras_[static_cast<size_t>(view)].GetCameraParams();
+    const int iteration = completed_ + 1;
+    const schedule::Step plan = schedule::StepFor(iteration, scene_extent_);
+    sh_degree_ = plan.sh_degree;
+    if (!renderer_.Render(gaussians_, camera, background_, image_, sh_degree_) ||
+        !solver_.Iterate(camera, renderer_, image_, targets_[static_cast<size_t>(view)], gaussians_,
+            plan.position_learning_rate, sh_degree_)) {
+        GS_NATIVE_ERROR("training iteration %d on view %d stopped: %s", iteration, view, gsn_last_error());
+        return false;
+    }
+    completed_ = iteration;
+    ++cycle_position_;
+    return Upkeep();
+}
+
+bool GaussianSplatting::Upkeep() {
+    const schedule::Upkeep upkeep = schedule::UpkeepAfter(completed_);
+    const int count = gaussians_.gaussian_count_;
+    if (upkeep.accumulate) {
+        const float* statistics = solver_.GetProjectionStatistics();
+        const float* absolute = solver_.GetAbsoluteProjectionStatistics();
+        if (!density_.AccumulateStatistics(statistics, absolute, count) ||
+            !density_.AccumulateFeatures(statistics, absolute, renderer_.GetDepths(),
+                reinterpret_cast<const float*>(renderer_.GetConicOpacities()), count)) {
+            return false;
+        }
+    }
+    if (upkeep.control_density) {
+        int changes[2] = {0, 0};
+        bool ok = density_.ControlDensity(gaussians_, scene_extent_, schedule::kGradientThreshold,
+                      schedule::kPercentDense, schedule::kMinOpacity, upkeep.max_screen_size, 0, changes,
+                      schedule::kAbsoluteGradientThreshold) &&
+                  solver_.RemapGaussians(density_.GetHistoryParentIds(), gaussians_.gaussian_count_);
+        if (!ok) {
+            GS_NATIVE_ERROR("density step after iteration %d stopped: %s", completed_, gsn_last_error());
+            return false;
+        }
+        GS_NATIVE_INFO("after iteration %d: %d -> %d Gaussians (+%d, -%d)", completed_, count,
+            gaussians_.gaussian_count_, changes[0], changes[1]);
+    }
+    if (upkeep.opacity_cap > 0.0f) {
+        if (!density_.ResetOpacity(gaussians_.gaussians_, gaussians_.gaussian_count_, upkeep.opacity_cap) ||
+            !solver_.ResetOpacityHistory()) {
+            return false;
+        }
+        GS_NATIVE_INFO("after iteration %d: opacity capped at %g", completed_, upkeep.opacity_cap);
+    }
     return true;
 }
 
 bool GaussianSplatting::Train(int iterations) {
-    if (iterations < 0 || !gaussians_.gaussians_ || scene_.train_cameras_.empty()) {
+    if (iterations < 0 || !gaussians_.gaussians_ || targets_.empty()) {
         return false;
     }
-    std::mt19937 generator(0);
-    for (int iteration = 0; iteration < iterations; ++iteration) {
-        if (gaussians_.gaussian_count_ == 0) {
+    shuffler_.seed(0);
+    for (int done = 0; done < iterations; ++done) {
+        if (gaussians_.gaussian_count_ == 0 || !Step()) {
             return false;
         }
-        if (camera_cursor_ >= static_cast<int>(camera_order_.size())) {
-            std::shuffle(camera_order_.begin(), camera_order_.end(), generator);
-            camera_cursor_ = 0;
-        }
-        int index = camera_order_[camera_cursor_];
-        CameraParams camera = scene_.train_cameras_[index].GetCameraParams();
-        int next = completed_ + 1;
-        sh_degree_ = std::min(3, next / 1000);
-        float fraction = std::min(1.0f, static_cast<float>(next) / 30000.0f);
-        float learning_rate =
-            scene_extent_ * std::exp(std::log(0.00016f) * (1.0f - fraction) + std::log(0.0000016f) * fraction);
-        if (!renderer_.Render(gaussians_, camera, background_, image_, sh_degree_)) {
-            GS_NATIVE_ERROR("Iteration %d: rendering failed.", next);
-            return false;
-        }
-        if (!solver_.Iterate(camera, renderer_, image_, targets_[index], gaussians_, learning_rate, sh_degree_)) {
-            GS_NATIVE_ERROR("Iteration %d: solver failed.", next);
-            return false;
-        }
-        completed_ = next;
-        ++camera_cursor_;
-        if (completed_ < 15000) {
-            if (!density_.AccumulateStatistics(solver_.GetProjectionStatistics(),
-                    solver_.GetAbsoluteProjectionStatistics(), gaussians_.gaussian_count_) ||
-                !density_.AccumulateFeatures(solver_.GetProjectionStatistics(),
-                    solver_.GetAbsoluteProjectionStatistics(), renderer_.GetDepths(),
-                    reinterpret_cast<const float*>(renderer_.GetConicOpacities()), gaussians_.gaussian_count_)) {
-                return false;
-            }
-            if (completed_ > 500 && completed_ % 100 == 0 && !ControlDensity()) {
-                return false;
-            }
-            if (completed_ % 500 == 0) {
-                float max_opacity = completed_ % 3000 == 0 ? 0.01f : 0.8f;
-                if (!density_.ResetOpacity(gaussians_.gaussians_, gaussians_.gaussian_count_, max_opacity) ||
-                    !solver_.ResetOpacityHistory()) {
-                    return false;
-                }
-            }
-        }
-        GS_NATIVE_INFO("Iteration %d/%d, completed %d, SH %d, gaussians %d.", iteration + 1, iterations, completed_,
-            sh_degree_, gaussians_.gaussian_count_);
     }
     return Check(gsn_context_synchronize(DefaultContext()), "gsn_context_synchronize");
 }
 
 bool GaussianSplatting::Render(const Camera& source, ImageF& image) {
-    if (!gaussians_.gaussians_ || source.image_size_.width <= 0 || source.image_size_.height <= 0) {
+    const int width = source.image_size_.width;
+    const int height = source.image_size_.height;
+    if (!gaussians_.gaussians_ || width <= 0 || height <= 0) {
         return false;
     }
-    CameraParams camera = source.GetCameraParams();
-    size_t pixels = static_cast<size_t>(camera.width_) * camera.height_;
-    if (pixels > pixel_capacity_) {
-        float* replacement = nullptr;
-        if (!DeviceAllocateCount(&replacement, pixels * 3)) {
-            return false;
-        }
-        DeviceFree(image_);
-        image_ = replacement;
-        pixel_capacity_ = pixels;
-    }
-    if (!renderer_.Render(gaussians_, camera, background_, image_, sh_degree_)) {
+    const size_t pixels = static_cast<size_t>(width) * height;
+    const CameraParams camera = source.GetCameraParams();
+    if (!EnsureImageCapacity(pixels) || !renderer_.Render(gaussians_, camera, background_, image_, sh_degree_) ||
+        !Check(gsn_context_synchronize(DefaultContext()), "gsn_context_synchronize")) {
         return false;
     }
-    image.width = camera.width_;
-    image.height = camera.height_;
+    image.width = width;
+    image.height = height;
     image.data.resize(pixels * 3);
-    return Check(gsn_context_synchronize(DefaultContext()), "gsn_context_synchronize") &&
-        gsn_memcpy_to_host(image.data.data(), image_, pixels * 3 * sizeof(float)) == GSN_OK;
+    return gsn_memcpy_to_host(image.data.data(), image_, pixels * 3 * sizeof(float)) == GSN_OK;
 }
 
 bool GaussianSplatting::Save(const std::string& path) {
-    int count = gaussians_.gaussian_count_;
+    const int count = gaussians_.gaussian_count_;
     if (count <= 0 || !gaussians_.gaussians_) {
         return false;
     }
-    std::vector<float> parameters(static_cast<size_t>(count) * kMaxParamCount);
+    std::vector<float> rows(static_cast<size_t>(count) * kMaxParamCount);
     if (!Check(gsn_context_synchronize(DefaultContext()), "gsn_context_synchronize") ||
-        gsn_memcpy_to_host(parameters.data(), gaussians_.gaussians_, parameters.size() * sizeof(float)) != GSN_OK) {
+        gsn_memcpy_to_host(rows.data(), gaussians_.gaussians_, rows.size() * sizeof(float)) != GSN_OK) {
         return false;
     }
-    std::vector<Gaussian> gaussians(count);
-    for (int id = 0; id < count; ++id) {
-        Gaussian& g = gaussians[id];
-        for (int axis = 0; axis < 3; ++axis) {
-            g.position_[axis] = parameters[static_cast<size_t>(axis) * count + id];
-            g.log_scale_[axis] = parameters[static_cast<size_t>(kLogScaleIndex + axis) * count + id];
-            g.rotation_[axis] = parameters[static_cast<size_t>(kRotationIndex + axis) * count + id];
-            for (int k = 0; k < 16; ++k) {
-                g.sh_(k, axis) = parameters[static_cast<size_t>(kShIndex + axis * 16 + k) * count + id];
-            }
-        }
-        g.opacity_logit_ = parameters[static_cast<size_t>(kOpacityIndex) * count + id];
-    }
-    return WriteGaussiansPly(path, gaussians);
-}
-
-bool GaussianSplatting::ControlDensity() {
-    int changes[2] = {};
-    if (!density_.ControlDensity(gaussians_, scene_extent_, 0.0002f, 0.01f, 0.005f, completed_ > 3000 ? 20.0f : 0.0f, 0,
-            changes, 0.0008f)) {
-        GS_NATIVE_ERROR("Iteration %d: density control failed.", completed_);
-        return false;
-    }
-    if (!solver_.RemapGaussians(density_.GetHistoryParentIds(), gaussians_.gaussian_count_)) {
-        GS_NATIVE_ERROR("Iteration %d: remapping Adam history after density control failed.", completed_);
-        return false;
-    }
-    GS_NATIVE_INFO("Iteration %d: density control, gaussians -> %d, added %d, removed %d.", completed_,
-        gaussians_.gaussian_count_, changes[0], changes[1]);
-    return true;
+    return WriteGaussiansPly(path, GaussiansFromRows(rows, count));
 }
 
 } // namespace gs_native

@@ -8,6 +8,7 @@
 #include "core/types/scene.h"
 
 #include <cstddef>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -41,8 +42,12 @@ public:
     }
 
 private:
-    bool InitializeGaussians();
-    bool ControlDensity();
+    // One training iteration on the next camera of the shuffled cycle, then the schedule's upkeep.
+    bool Step();
+    bool Upkeep();
+    int NextCamera();
+    bool EnsureImageCapacity(size_t pixels);
+    void ReleaseImages();
 
     Scene scene_;
     GaussianBuffers gaussians_;
@@ -51,13 +56,13 @@ private:
     Solver solver_;
     DensityController density_;
 
-    float* image_ = nullptr;
-    uint8_t* target_ = nullptr;
-    std::vector<uint8_t*> targets_;
-    size_t pixel_capacity_ = 0;
+    float* image_ = nullptr;             // device HWC float render target
+    size_t image_pixels_ = 0;
+    std::vector<uint8_t*> targets_;      // device copies of the training images, uploaded once
 
-    std::vector<int> camera_order_;
-    int camera_cursor_ = 0;
+    std::vector<int> cycle_;             // camera order of the current pass over the views
+    size_t cycle_position_ = 0;
+    std::mt19937 shuffler_;              // restarted from seed 0 by every Train call
 
     int completed_ = 0;
     int sh_degree_ = 0;

@@ -171,6 +171,14 @@ int gsn_image_gradients(gsn_context* context, int width, int height, const float
 int gsn_image_metrics(gsn_context* context, int width, int height, const float* image,
     const uint8_t* target, double* out);
 
+/* Held-out evaluation of a render (replaces the scoring of reference demo/render.cpp:130-140):
+ * the image is clamped to [0, 1], scaled by 255 in FP32 and rounded half to even to 8 bits
+ * (written to `quantized`, HWC, when not null), and compared with the 8-bit target.
+ * out[0] = sum of squared differences over all pixels and channels (exact),
+ * out[1] = PSNR in dB, -10 log10(out[0] / (3 W H 255^2)), +inf when out[0] = 0. Synchronises. */
+int gsn_image_quantized_error(gsn_context* context, int width, int height, const float* image,
+    const uint8_t* target, uint8_t* quantized, double* out);
+
 /* Allocation-free gsn_image_metrics: no counterpart in the reference. For callers that cannot
  * allocate device memory (code under HIP graph capture, evaluation loops that must not call
  * hipMalloc). Same kernels and results (bit for bit) as gsn_image_metrics, but the per-block
