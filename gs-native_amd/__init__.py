@@ -81,6 +81,7 @@ _PROTOTYPES = {
     "gsn_context_create": [ctypes.POINTER(_P)],
     "gsn_context_destroy": [_P],
     "gsn_context_synchronize": [_P],
+    "gsn_device_synchronize": [],
     "gsn_context_stream": [_P],
     "gsn_context_scratch_bytes": [_P],
     "gsn_status_string": [_I],
@@ -107,11 +108,14 @@ _PROTOTYPES = {
     "gsn_image_metrics_static": [_P, _I, _I, _P, _P, _P],
     "gsn_image_quantized_error": [_P, _I, _I, _P, _P, _P, ctypes.POINTER(ctypes.c_double)],
     "gsn_gaussian_gradients": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
-    "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+    "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
                                     _F, _F, _F, _P, _P],
-    "gsn_adam_update": [_P, _I, _P, _P, _P, _P, _F, _F, _F],
-    "gsn_remap_history": [_P, _P, _I, _I, _P, _P, _P, _P],
-    "gsn_zero_rows": [_P, _P, _P, _I, _I, _I],
+    "gsn_adam_update": [_P, _I, _P, _P, _P, _F, _F, _F],
+    "gsn_remap_history": [_P, _P, _I, _I, _P, _P],
+    "gsn_zero_rows": [_P, _P, _I, _I, _I],
+    "gsn_moment_floats": [_I],
+    "gsn_moments_unpack": [_P, _I, _P, _P, _P],
+    "gsn_moments_pack": [_P, _I, _P, _P, _P],
     "gsn_density_count": [_P, _P, _I, _P, _P, _P, _F, _F, _F, _F, _I, _F, _P, _P, ctypes.POINTER(ctypes.c_int32)],
     "gsn_density_write": [_P, _P, _I, _P, _P, ctypes.POINTER(ctypes.c_int32), _U64, _U64, _P, _I, _P, _P, _P],
     "gsn_density_normals": [_P, _U64, _U64, _I64, _P],
@@ -130,7 +134,7 @@ _PROTOTYPES = {
     "gsn_gaussian_gradients_stored": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
                                       ctypes.POINTER(_F), _P, _P, _P, _P, _P],
     "gsn_gaussian_gradients_adam_stored": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
-                                           ctypes.POINTER(_F), _P, _P, _P, _F, _F, _F, _P, _P],
+                                           ctypes.POINTER(_F), _P, _P, _F, _F, _F, _P, _P],
     "gsn_mlp_sizes": [ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "gsn_mlp_sync_weights": [_P, _P, _P],
     "gsn_mlp_forward": [_P, _P, _I, _P, _P, _P],
@@ -151,6 +155,7 @@ _RESTYPES = {
     "gsn_context_destroy": None,
     "gsn_context_stream": _P,
     "gsn_context_scratch_bytes": ctypes.c_size_t,
+    "gsn_moment_floats": ctypes.c_size_t,
     "gsn_status_string": ctypes.c_char_p,
     "gsn_last_error": ctypes.c_char_p,
 }
@@ -244,6 +249,12 @@ class DeviceBuffer:
             call("gsn_memcpy_to_host", out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes)
         return out
 
+    def view(self, shape, dtype=None) -> "DeviceBuffer":
+        """A buffer of `shape` over the start of this allocation (no copy, no ownership): lets a
+        capacity-sized allocation serve a smaller logical array, so that counts that change by a
+        little (density control) do not reallocate."""
+        return _DeviceView(self, shape, self.dtype if dtype is None else dtype)
+
     def free(self) -> None:
         if self._pointer.value:
             lib.gsn_device_free(self._pointer)
@@ -254,6 +265,41 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class _DeviceView(DeviceBuffer):
+    def __init__(self, owner: DeviceBuffer, shape, dtype):
+        self.shape = tuple(int(v) for v in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        if self.nbytes > owner.nbytes:
+            raise ValueError(f"a view of {self.nbytes} bytes exceeds its allocation of {owner.nbytes}")
+        self._owner = owner
+        self._pointer = owner.ptr
+
+    def free(self) -> None:
+        self._pointer = ctypes.c_void_p(None)
+        self._owner = None
+
+
+class CapacityBuffer:
+    """An allocation that grows by 25% when a larger array is asked for and otherwise hands out
+    views of the requested shape."""
+
+    def __init__(self, dtype):
+        self.dtype = np.dtype(dtype)
+        self.storage = None
+
+    def get(self, shape, zero: bool = False) -> DeviceBuffer:
+        shape = tuple(int(v) for v in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        needed = max(int(np.prod(shape, dtype=np.int64)), 1)
+        if self.storage is None or self.storage.shape[0] < needed:
+            self.storage = None
+            self.storage = DeviceBuffer((needed + needed // 4,), self.dtype)
+        view = self.storage.view(shape)
+        if zero and view.nbytes:
+            call("gsn_memset_device", view.ptr, 0, view.nbytes)
+        return view
 
 
 class Context:
@@ -336,28 +382,36 @@ class Event:
 def _p(buffer) -> ctypes.c_void_p:
     if buffer is None:
         return ctypes.c_void_p(None)
-    if isinstance(buffer, DeviceBuffer):
+    if isinstance(buffer, (DeviceBuffer, Moments)):
         return buffer.ptr
     return ctypes.c_void_p(int(buffer))
 
 
 class Projection:
-    """Device outputs of projection + tile lists for one view (the Renderer's getters)."""
+    """Device outputs of projection + tile lists for one view (the Renderer's getters). The
+    per-Gaussian buffers are views of capacity allocations, so resize() to a nearby count (after
+    density control) reuses them."""
 
     def __init__(self, count: int, camera: Camera):
-        self.count = count
         self.camera = camera
-        n = max(count, 1)
-        self.means = DeviceBuffer((n, 2), np.float32)
-        self.conic_opacities = DeviceBuffer((n, 4), np.float32)
-        self.colors = DeviceBuffer((n, 3), np.float32)
-        self.depths = DeviceBuffer((n,), np.float32)
-        self.tile_rects = DeviceBuffer((n, 4), np.int32)
-        self.record_offsets = DeviceBuffer((count + 1,), np.int32)
+        self._stores = {name: CapacityBuffer(dtype) for name, dtype in (
+            ("means", np.float32), ("conic_opacities", np.float32), ("colors", np.float32), ("depths", np.float32),
+            ("tile_rects", np.int32), ("record_offsets", np.int32))}
+        self.resize(count)
         self.tile_offsets = DeviceBuffer((camera.tile_count + 1,), np.int64)
         self.record_count = 0
         self.tile_gaussian_ids = DeviceBuffer((1,), np.int32)
         self.sorted_keys = None
+
+    def resize(self, count: int) -> None:
+        self.count = count
+        n = max(count, 1)
+        self.means = self._stores["means"].get((n, 2))
+        self.conic_opacities = self._stores["conic_opacities"].get((n, 4))
+        self.colors = self._stores["colors"].get((n, 3))
+        self.depths = self._stores["depths"].get((n,))
+        self.tile_rects = self._stores["tile_rects"].get((n, 4))
+        self.record_offsets = self._stores["record_offsets"].get((count + 1,))
 
 
 def render(context: Context, params: DeviceBuffer, count: int, camera: Camera, background, image: DeviceBuffer,
@@ -366,8 +420,10 @@ def render(context: Context, params: DeviceBuffer, count: int, camera: Camera, b
     """Projection -> record scan -> tile lists -> raster forward (Renderer::Render).
     stored_state = (final_transmittance, last_contributor) selects the forward that also writes the
     per-pixel state of the stored-state backward."""
-    if projection is None or projection.count != count:
+    if projection is None:
         projection = Projection(count, camera)
+    elif projection.count != count:
+        projection.resize(count)
     projection.camera = camera
     if projection.tile_offsets.shape[0] < camera.tile_count + 1:
         projection.tile_offsets = DeviceBuffer((camera.tile_count + 1,), np.int64)
@@ -432,14 +488,63 @@ def gaussian_gradients(context: Context, projection: Projection, params: DeviceB
          _p(statistics), _p(absolute_statistics))
 
 
+class Moments:
+    """Adam moments of `count` Gaussians in the library's blocked layout (gsn_moment_floats
+    floats: per 64 Gaussians and row, 64 (first, second) pairs). numpy() / from_numpy() convert
+    to and from two [58][count] arrays on the device. The layout of a Gaussian does not depend on
+    the count, so a buffer allocated for a larger count serves a smaller one (store)."""
+
+    def __init__(self, count: int, store: "CapacityBuffer | None" = None, zero: bool = True):
+        self.count = int(count)
+        shape = (max(int(lib.gsn_moment_floats(self.count)), 1),)
+        self.buffer = store.get(shape, zero=zero) if store is not None else (
+            DeviceBuffer.zeros(shape, np.float32) if zero else DeviceBuffer(shape, np.float32))
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self.buffer.ptr
+
+    @classmethod
+    def from_numpy(cls, first: np.ndarray, second: np.ndarray) -> "Moments":
+        first = np.ascontiguousarray(first, np.float32)
+        moments = cls(first.shape[1])
+        if moments.count:
+            first_device = DeviceBuffer.from_numpy(first)
+            second_device = DeviceBuffer.from_numpy(np.ascontiguousarray(second, np.float32))
+            call("gsn_device_synchronize")
+            call("gsn_moments_pack", _default_context().handle, moments.count, first_device.ptr, second_device.ptr,
+                 moments.ptr)
+            _default_context().synchronize()
+        return moments
+
+    def numpy(self) -> tuple[np.ndarray, np.ndarray]:
+        first = DeviceBuffer((PARAM_COUNT, max(self.count, 1)), np.float32)
+        second = DeviceBuffer((PARAM_COUNT, max(self.count, 1)), np.float32)
+        if self.count:
+            call("gsn_device_synchronize")  # the moments may still be written on another context's stream
+            call("gsn_moments_unpack", _default_context().handle, self.count, self.ptr, first.ptr, second.ptr)
+            _default_context().synchronize()
+        return first.numpy()[:, : self.count], second.numpy()[:, : self.count]
+
+
+_CONVERSION_CONTEXT = []
+
+
+def _default_context() -> "Context":
+    """A context of its own for moment conversions (synchronised after each use)."""
+    if not _CONVERSION_CONTEXT:
+        _CONVERSION_CONTEXT.append(Context())
+    return _CONVERSION_CONTEXT[0]
+
+
 def gaussian_gradients_adam(context: Context, projection: Projection, params: DeviceBuffer, sh_degree: int,
-                            image: DeviceBuffer, image_grad: DeviceBuffer, first: DeviceBuffer, second: DeviceBuffer,
+                            image: DeviceBuffer, image_grad: DeviceBuffer, moments: Moments,
                             position_lr: float, first_correction: float, second_correction: float,
                             statistics: DeviceBuffer, absolute_statistics: DeviceBuffer) -> None:
     p = projection
     call("gsn_gaussian_gradients_adam", context.handle, ctypes.byref(p.camera), p.count, sh_degree, _p(params),
          _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets), _p(p.tile_rects),
-         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(first), _p(second),
+         _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(moments),
          position_lr, first_correction, second_correction, _p(statistics), _p(absolute_statistics))
 
 
@@ -450,10 +555,10 @@ def adam_corrections(step: int) -> tuple[float, float]:
     return float(first), float(second)
 
 
-def adam_update(context: Context, count: int, gradients: DeviceBuffer, params: DeviceBuffer, first: DeviceBuffer,
-                second: DeviceBuffer, position_lr: float, step: int) -> None:
+def adam_update(context: Context, count: int, gradients: DeviceBuffer, params: DeviceBuffer, moments: Moments,
+                position_lr: float, step: int) -> None:
     c1, c2 = adam_corrections(step)
-    call("gsn_adam_update", context.handle, count, _p(gradients), _p(params), _p(first), _p(second), position_lr, c1, c2)
+    call("gsn_adam_update", context.handle, count, _p(gradients), _p(params), _p(moments), position_lr, c1, c2)
 
 
 def knn3(context: Context, points: np.ndarray) -> tuple[np.ndarray, np.ndarray]:

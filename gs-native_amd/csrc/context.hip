@@ -139,6 +139,10 @@ void gsn_context_destroy(gsn_context* context) {
     delete context;
 }
 
+int gsn_device_synchronize(void) {
+    return gsn::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
 int gsn_context_synchronize(gsn_context* context) {
     if (!context) {
         return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_context_synchronize: null context");

@@ -271,13 +271,16 @@ __device__ __forceinline__ void adam_values(float& param, float& first, float& s
     param -= learning_rate * (m * first_correction) / (sqrtf(v * second_correction) + 1e-15f);
 }
 
-__device__ __forceinline__ void adam_entry(float* params, float* first, float* second, int64_t index, float gradient,
-    float learning_rate, float first_correction, float second_correction) {
-    float p = params[index], m = first[index], v = second[index];
-    adam_values(p, m, v, gradient, learning_rate, first_correction, second_correction);
-    first[index] = m;
-    second[index] = v;
-    params[index] = p;
+// Two-float vector for non-temporal moment pairs (the builtins take native vector types).
+typedef float moment_pair __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ moment_pair load_moments(const float* moments, int64_t index) {
+    return __builtin_nontemporal_load(reinterpret_cast<const moment_pair*>(moments + index));
+}
+
+__device__ __forceinline__ void store_moments(float* moments, int64_t index, float m, float v) {
+    moment_pair pair = {m, v};
+    __builtin_nontemporal_store(pair, reinterpret_cast<moment_pair*>(moments + index));
 }
 
 constexpr int kAdamGroup = 16;
@@ -415,7 +418,7 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     const float* __restrict__ partials,
     const int* __restrict__ record_offsets, const uint8_t* __restrict__ record_flags, uint8_t epoch, int count,
     int record_count, Camera cam, float* params, float* gradients, float* screen, float* statistics,
-    float* absolute_statistics, float* first, float* second, float position_learning_rate, float first_correction,
+    float* absolute_statistics, float* moments, float position_learning_rate, float first_correction,
     float second_correction) {
     __shared__ float4 s_records[kGradientBlock / 64][kChunks * 64 * 3];
     const int wave = threadIdx.x >> 6;
@@ -464,25 +467,25 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     };
     if (Fused) {
         // Adam over the 58 rows in groups: all loads of a group are issued before its stores, so
-        // each wave keeps 3 * kAdamGroup loads in flight instead of one row's. With line-aligned
-        // rows (Streaming) they are streamed with non-temporal loads and stores, which keep them
-        // from displacing cached data (measured 1.28 -> 1.15 ms at 3M Gaussians). With rows off
-        // the line grid (a count that is not a multiple of 32, as after density control) each
-        // wave's row segment straddles three lines and non-temporal loads fetch the shared end
-        // lines twice (1.84 ms); ordinary accesses then take 1.28 ms.
+        // each wave keeps 2 * kAdamGroup loads in flight instead of one row's. The moments are
+        // blocked (a wave's 64 pairs of a row are one aligned 512-byte piece), so they always
+        // stream with non-temporal accesses that keep them from displacing cached data. The
+        // parameter rows are the interface's [58][N] layout: they stream non-temporally only when
+        // every row starts on a 128-byte line (Streaming); off the line grid (a count that is not
+        // a multiple of 32, as after density control) each wave's row segment straddles three
+        // lines, which ordinary accesses share with the neighbouring wave through the cache.
         float* __restrict__ p = params;
-        float* __restrict__ m = first;
-        float* __restrict__ v = second;
+        float* __restrict__ mv = moments;
 #pragma unroll
         for (int base = 0; base < kParamCount; base += kAdamGroup) {
-            float pv[kAdamGroup], mv[kAdamGroup], vv[kAdamGroup];
+            float pv[kAdamGroup];
+            moment_pair qv[kAdamGroup];
 #pragma unroll
             for (int k = 0; k < kAdamGroup; ++k) {
                 if (base + k < kParamCount) {
                     int64_t index = (base + k) * n + id;
                     pv[k] = Streaming ? __builtin_nontemporal_load(p + index) : p[index];
-                    mv[k] = Streaming ? __builtin_nontemporal_load(m + index) : m[index];
-                    vv[k] = Streaming ? __builtin_nontemporal_load(v + index) : v[index];
+                    qv[k] = load_moments(mv, moment_index(id, base + k));
                 }
             }
 #pragma unroll
@@ -490,26 +493,16 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
                 if (base + k < kParamCount) {
                     int row = base + k;
                     int64_t index = row * n + id;
-                    adam_values(pv[k], mv[k], vv[k], row_gradient(row),
-                        learning_rate_for_row(row, position_learning_rate), first_correction, second_correction);
+                    float m = qv[k].x, v = qv[k].y;
+                    adam_values(pv[k], m, v, row_gradient(row), learning_rate_for_row(row, position_learning_rate),
+                        first_correction, second_correction);
                     if (Streaming) {
                         __builtin_nontemporal_store(pv[k], p + index);
                     }
                     else {
                         p[index] = pv[k];
                     }
-                    if (Streaming) {
-                        __builtin_nontemporal_store(mv[k], m + index);
-                    }
-                    else {
-                        m[index] = mv[k];
-                    }
-                    if (Streaming) {
-                        __builtin_nontemporal_store(vv[k], v + index);
-                    }
-                    else {
-                        v[index] = vv[k];
-                    }
+                    store_moments(mv, moment_index(id, row), m, v);
                 }
             }
         }
@@ -522,43 +515,70 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     }
 }
 
-__global__ __launch_bounds__(256) void adam_kernel(const float* __restrict__ gradients, float* params, float* first,
-    float* second, int count, float position_learning_rate, float first_correction, float second_correction) {
+__global__ __launch_bounds__(256) void adam_kernel(const float* __restrict__ gradients, float* params, float* moments,
+    int count, float position_learning_rate, float first_correction, float second_correction) {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= count) {
         return;
     }
     int row = blockIdx.y;
     int64_t index = static_cast<int64_t>(row) * count + id;
-    adam_entry(params, first, second, index, gradients[index], learning_rate_for_row(row, position_learning_rate),
-        first_correction, second_correction);
+    int64_t pair = moment_index(id, row);
+    float p = params[index];
+    moment_pair q = load_moments(moments, pair);
+    float m = q.x, v = q.y;
+    adam_values(p, m, v, gradients[index], learning_rate_for_row(row, position_learning_rate), first_correction,
+        second_correction);
+    params[index] = p;
+    store_moments(moments, pair, m, v);
 }
 
-__global__ __launch_bounds__(256) void remap_history_kernel(const int* parent_ids, const float* previous_first,
-    const float* previous_second, float* first, float* second, int previous_count, int count) {
+__global__ __launch_bounds__(256) void remap_history_kernel(const int* parent_ids, const float* previous, float* moments,
+    int previous_count, int count) {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= count) {
         return;
     }
     int row = blockIdx.y;
     int parent = parent_ids ? parent_ids[id] : -1;
-    int64_t index = static_cast<int64_t>(row) * count + id;
+    float2 pair = make_float2(0.0f, 0.0f);
     if (parent >= 0 && parent < previous_count) {
-        int64_t previous = static_cast<int64_t>(row) * previous_count + parent;
-        first[index] = previous_first[previous];
-        second[index] = previous_second[previous];
+        pair = *reinterpret_cast<const float2*>(previous + moment_index(parent, row));
     }
-    else {
-        first[index] = 0.0f;
-        second[index] = 0.0f;
-    }
+    *reinterpret_cast<float2*>(moments + moment_index(id, row)) = pair;
 }
 
-__global__ __launch_bounds__(256) void zero_rows_kernel(float* first, float* second, int64_t begin, int64_t total) {
+// Blocked moments <-> two [58][count] arrays (first and second moments).
+__global__ __launch_bounds__(256) void moments_unpack_kernel(const float* moments, float* first, float* second,
+    int count) {
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count) {
+        return;
+    }
+    int row = blockIdx.y;
+    float2 pair = *reinterpret_cast<const float2*>(moments + moment_index(id, row));
+    first[static_cast<int64_t>(row) * count + id] = pair.x;
+    second[static_cast<int64_t>(row) * count + id] = pair.y;
+}
+
+__global__ __launch_bounds__(256) void moments_pack_kernel(const float* first, const float* second, float* moments,
+    int count) {
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count) {
+        return;
+    }
+    int row = blockIdx.y;
+    *reinterpret_cast<float2*>(moments + moment_index(id, row)) =
+        make_float2(first[static_cast<int64_t>(row) * count + id], second[static_cast<int64_t>(row) * count + id]);
+}
+
+// Zeroes the moment pairs of rows [row, row + rows) for all `count` Gaussians.
+__global__ __launch_bounds__(256) void zero_rows_kernel(float* moments, int count, int row, int rows) {
     int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < total) {
-        first[begin + i] = 0.0f;
-        second[begin + i] = 0.0f;
+    if (i < static_cast<int64_t>(count) * rows) {
+        int r = row + static_cast<int>(i / count);
+        int64_t id = i % count;
+        *reinterpret_cast<float2*>(moments + moment_index(id, r)) = make_float2(0.0f, 0.0f);
     }
 }
 
@@ -566,60 +586,60 @@ template<bool Fused>
 int launch_gradients(gsn_context* context, const Camera& cam, int count, int record_count, int sh_degree,
     const int* group_order, const float* partials, const int* record_offsets, const uint8_t* record_flags,
     uint8_t epoch, float* params, float* gradients, float* screen, float* statistics, float* absolute_statistics,
-    float* first, float* second, float lr, float c1, float c2) {
+    float* moments, float lr, float c1, float c2) {
     unsigned grid = grid_for(count, kGradientBlock);
     hipStream_t stream = context->stream;
-    // Non-temporal streaming of the Adam rows only when all of them start on a 128-byte line.
-    const bool streaming = Fused && rows_line_aligned(params, count) && rows_line_aligned(first, count) &&
-                           rows_line_aligned(second, count);
+    // Non-temporal streaming of the parameter rows only when all of them start on a 128-byte line
+    // (the blocked moments always stream non-temporally).
+    const bool streaming = Fused && rows_line_aligned(params, count);
     StageScope scope(context, kStageGaussianGradients);
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
         if (streaming) {
             gaussian_gradients_kernel<0, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
                 record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, first, second, lr, c1, c2);
+                absolute_statistics, moments, lr, c1, c2);
         }
         else {
             gaussian_gradients_kernel<0, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
                 record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, first, second, lr, c1, c2);
+                absolute_statistics, moments, lr, c1, c2);
         }
         break;
     case 1:
         if (streaming) {
             gaussian_gradients_kernel<1, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
                 record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, first, second, lr, c1, c2);
+                absolute_statistics, moments, lr, c1, c2);
         }
         else {
             gaussian_gradients_kernel<1, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
                 record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, first, second, lr, c1, c2);
+                absolute_statistics, moments, lr, c1, c2);
         }
         break;
     case 2:
         if (streaming) {
             gaussian_gradients_kernel<2, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
                 record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, first, second, lr, c1, c2);
+                absolute_statistics, moments, lr, c1, c2);
         }
         else {
             gaussian_gradients_kernel<2, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
                 record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, first, second, lr, c1, c2);
+                absolute_statistics, moments, lr, c1, c2);
         }
         break;
     default:
         if (streaming) {
             gaussian_gradients_kernel<3, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
                 record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, first, second, lr, c1, c2);
+                absolute_statistics, moments, lr, c1, c2);
         }
         else {
             gaussian_gradients_kernel<3, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
                 record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, first, second, lr, c1, c2);
+                absolute_statistics, moments, lr, c1, c2);
         }
         break;
     }
@@ -678,15 +698,15 @@ extern "C" int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* ca
     }
     return launch_gradients<false>(context, cam, count, record_count, sh_degree, group_order, partials,
         record_offsets, flags, epoch, const_cast<float*>(params), gradients, screen_gradients, statistics,
-        absolute_statistics, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
+        absolute_statistics, nullptr, 0.0f, 0.0f, 0.0f);
 }
 
 extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, int count, int sh_degree,
     float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
     const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
-    const float* colors, const float* image, const float* image_gradients, float* first_moments,
-    float* second_moments, float position_learning_rate, float first_correction, float second_correction,
-    float* statistics, float* absolute_statistics) {
+    const float* colors, const float* image, const float* image_gradients, float* moments,
+    float position_learning_rate, float first_correction, float second_correction, float* statistics,
+    float* absolute_statistics) {
     using namespace gsn;
     GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
         record_offsets, tile_rects, means, conic_opacities, colors, image, image_gradients, statistics,
@@ -694,7 +714,7 @@ extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camer
     if (count == 0) {
         return GSN_OK;
     }
-    if (!first_moments || !second_moments || !(position_learning_rate >= 0.0f) || !std::isfinite(position_learning_rate)) {
+    if (!moments || !(position_learning_rate >= 0.0f) || !std::isfinite(position_learning_rate)) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_gaussian_gradients_adam: invalid optimiser arguments");
     }
     Camera cam;
@@ -713,8 +733,8 @@ extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camer
             &group_order));
     }
     return launch_gradients<true>(context, cam, count, record_count, sh_degree, group_order, partials,
-        record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, first_moments,
-        second_moments, position_learning_rate, first_correction, second_correction);
+        record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, moments,
+        position_learning_rate, first_correction, second_correction);
 }
 
 extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_camera* camera, int count,
@@ -753,15 +773,15 @@ extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_cam
     }
     return launch_gradients<false>(context, cam, count, record_count, sh_degree, group_order, partials,
         record_offsets, flags, epoch, const_cast<float*>(params), gradients, screen_gradients, statistics,
-        absolute_statistics, nullptr, nullptr, 0.0f, 0.0f, 0.0f);
+        absolute_statistics, nullptr, 0.0f, 0.0f, 0.0f);
 }
 
 extern "C" int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gsn_camera* camera, int count,
     int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
     const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
     const float* colors, const float* final_transmittance, const int32_t* last_contributor, const float* background,
-    const float* image_gradients, float* first_moments, float* second_moments, float position_learning_rate,
-    float first_correction, float second_correction, float* statistics, float* absolute_statistics) {
+    const float* image_gradients, float* moments, float position_learning_rate, float first_correction,
+    float second_correction, float* statistics, float* absolute_statistics) {
     using namespace gsn;
     GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
         record_offsets, tile_rects, means, conic_opacities, colors, image_gradients, image_gradients, statistics,
@@ -769,7 +789,7 @@ extern "C" int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gs
     if (count == 0) {
         return GSN_OK;
     }
-    if (!final_transmittance || !last_contributor || !background || !first_moments || !second_moments ||
+    if (!final_transmittance || !last_contributor || !background || !moments ||
         !(position_learning_rate >= 0.0f) || !std::isfinite(position_learning_rate)) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_gaussian_gradients_adam_stored: invalid arguments");
     }
@@ -791,13 +811,12 @@ extern "C" int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gs
             &stored));
     }
     return launch_gradients<true>(context, cam, count, record_count, sh_degree, group_order, partials,
-        record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, first_moments,
-        second_moments, position_learning_rate, first_correction, second_correction);
+        record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, moments,
+        position_learning_rate, first_correction, second_correction);
 }
 
 extern "C" int gsn_adam_update(gsn_context* context, int count, const float* gradients, float* params,
-    float* first_moments, float* second_moments, float position_learning_rate, float first_correction,
-    float second_correction) {
+    float* moments, float position_learning_rate, float first_correction, float second_correction) {
     using namespace gsn;
     if (!context || count < 0 || !(position_learning_rate >= 0.0f) || !std::isfinite(position_learning_rate)) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_adam_update: invalid arguments");
@@ -805,18 +824,18 @@ extern "C" int gsn_adam_update(gsn_context* context, int count, const float* gra
     if (count == 0) {
         return GSN_OK;
     }
-    if (!gradients || !params || !first_moments || !second_moments) {
+    if (!gradients || !params || !moments) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_adam_update: null buffer");
     }
     dim3 grid(grid_for(count, 256), kParamCount);
     StageScope scope(context, kStageAdam);
-    adam_kernel<<<grid, 256, 0, context->stream>>>(gradients, params, first_moments, second_moments, count,
-        position_learning_rate, first_correction, second_correction);
+    adam_kernel<<<grid, 256, 0, context->stream>>>(gradients, params, moments, count, position_learning_rate,
+        first_correction, second_correction);
     return check_launch("adam_kernel");
 }
 
 extern "C" int gsn_remap_history(gsn_context* context, const int* parent_ids, int previous_count, int count,
-    const float* previous_first, const float* previous_second, float* first, float* second) {
+    const float* previous_moments, float* moments) {
     using namespace gsn;
     if (!context || count < 0 || previous_count < 0) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_remap_history: invalid arguments");
@@ -824,29 +843,60 @@ extern "C" int gsn_remap_history(gsn_context* context, const int* parent_ids, in
     if (count == 0) {
         return GSN_OK;
     }
-    if (!first || !second || (previous_count > 0 && parent_ids && (!previous_first || !previous_second))) {
-        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_remap_history: null buffer");
+    if (!moments || (previous_count > 0 && parent_ids && !previous_moments) || previous_moments == moments) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_remap_history: null or aliased moment buffer");
     }
     dim3 grid(grid_for(count, 256), kParamCount);
     StageScope scope(context, kStageDensity);
-    remap_history_kernel<<<grid, 256, 0, context->stream>>>(parent_ids, previous_first, previous_second, first, second,
-        previous_first ? previous_count : 0, count);
+    remap_history_kernel<<<grid, 256, 0, context->stream>>>(parent_ids, previous_moments, moments,
+        previous_moments ? previous_count : 0, count);
     return check_launch("remap_history_kernel");
 }
 
-extern "C" int gsn_zero_rows(gsn_context* context, float* first, float* second, int stride, int row, int rows) {
+extern "C" int gsn_zero_rows(gsn_context* context, float* moments, int count, int row, int rows) {
     using namespace gsn;
-    if (!context || stride < 0 || row < 0 || rows < 0 || row + rows > kParamCount) {
+    if (!context || count < 0 || row < 0 || rows < 0 || row + rows > kParamCount) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_zero_rows: invalid arguments");
     }
-    int64_t total = static_cast<int64_t>(stride) * rows;
+    int64_t total = static_cast<int64_t>(count) * rows;
     if (total == 0) {
         return GSN_OK;
     }
-    if (!first || !second) {
+    if (!moments) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_zero_rows: null buffer");
     }
-    zero_rows_kernel<<<grid_for(total, 256), 256, 0, context->stream>>>(first, second, static_cast<int64_t>(row) * stride,
-        total);
+    zero_rows_kernel<<<grid_for(total, 256), 256, 0, context->stream>>>(moments, count, row, rows);
     return check_launch("zero_rows_kernel");
+}
+
+extern "C" size_t gsn_moment_floats(int count) {
+    return count > 0 ? static_cast<size_t>(gsn::moment_floats(count)) : 0;
+}
+
+extern "C" int gsn_moments_unpack(gsn_context* context, int count, const float* moments, float* first,
+    float* second) {
+    using namespace gsn;
+    if (!context || count < 0 || (count > 0 && (!moments || !first || !second))) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_moments_unpack: invalid arguments");
+    }
+    if (count == 0) {
+        return GSN_OK;
+    }
+    dim3 grid(grid_for(count, 256), kParamCount);
+    moments_unpack_kernel<<<grid, 256, 0, context->stream>>>(moments, first, second, count);
+    return check_launch("moments_unpack_kernel");
+}
+
+extern "C" int gsn_moments_pack(gsn_context* context, int count, const float* first, const float* second,
+    float* moments) {
+    using namespace gsn;
+    if (!context || count < 0 || (count > 0 && (!moments || !first || !second))) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_moments_pack: invalid arguments");
+    }
+    if (count == 0) {
+        return GSN_OK;
+    }
+    dim3 grid(grid_for(count, 256), kParamCount);
+    moments_pack_kernel<<<grid, 256, 0, context->stream>>>(first, second, moments, count);
+    return check_launch("moments_pack_kernel");
 }

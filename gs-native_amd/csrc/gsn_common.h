@@ -31,6 +31,25 @@ constexpr int kShIndex = 10;
 constexpr int kParamCount = 58;
 constexpr int kShCoeffCount = 16;
 
+// Adam moments are the optimiser's own state, not part of the reference's interface, and are
+// stored blocked: block b holds Gaussians [64 b, 64 b + 64), and inside it each of the 58 rows
+// holds 64 (first, second) pairs. A wave of 64 consecutive Gaussians then reads and writes one
+// contiguous, line-aligned 512-byte piece per row whatever the Gaussian count, and one row of
+// both moments is one stream instead of two.
+constexpr int kMomentBlock = 64;
+constexpr int64_t kMomentBlockFloats = static_cast<int64_t>(kParamCount) * kMomentBlock * 2;
+
+// Floats of the blocked moment buffer of `count` Gaussians.
+GSN_HD int64_t moment_floats(int64_t count) {
+    return (count + kMomentBlock - 1) / kMomentBlock * kMomentBlockFloats;
+}
+
+// Float index of the first moment of Gaussian i in row `row` (the second moment follows it).
+GSN_HD int64_t moment_index(int64_t i, int row) {
+    return (i / kMomentBlock) * kMomentBlockFloats + static_cast<int64_t>(row) * (kMomentBlock * 2) +
+           (i % kMomentBlock) * 2;
+}
+
 // Renderer constants (reference src/core/render/renderer.cu:11-15).
 constexpr int kTileSize = 16;
 constexpr float kLowPassVariance = 0.3f;

@@ -51,8 +51,7 @@ def measure(form: str, gaussians: int, width: int, height: int, steps: int) -> d
     target = gsn.DeviceBuffer.from_numpy(np.clip(np.rint(image.numpy() * 255.0), 0, 255).astype(np.uint8))
     del truth, projection
     device_params = gsn.DeviceBuffer.from_numpy(scenes.perturb(params, 6, 0.01))
-    first = gsn.DeviceBuffer.zeros((58, gaussians), np.float32)
-    second = gsn.DeviceBuffer.zeros((58, gaussians), np.float32)
+    moments = gsn.Moments(gaussians)
     stats = gsn.DeviceBuffer.zeros((2, gaussians), np.float32)
     abs_stats = gsn.DeviceBuffer.zeros((2, gaussians), np.float32)
     image_grad = gsn.DeviceBuffer((height, width, 3), np.float32)
@@ -71,13 +70,13 @@ def measure(form: str, gaussians: int, width: int, height: int, steps: int) -> d
         c1, c2 = gsn.adam_corrections(step + 1)
         p = projection
         if state is None:
-            gsn.gaussian_gradients_adam(context, p, device_params, 3, image, image_grad, first, second, 1e-4, c1, c2,
+            gsn.gaussian_gradients_adam(context, p, device_params, 3, image, image_grad, moments, 1e-4, c1, c2,
                                         stats, abs_stats)
         else:
             gsn.call("gsn_gaussian_gradients_adam_stored", context.handle, ctypes.byref(p.camera), p.count, 3,
                      device_params.ptr, p.tile_gaussian_ids.ptr, p.tile_offsets.ptr, p.record_count,
                      p.record_offsets.ptr, p.tile_rects.ptr, p.means.ptr, p.conic_opacities.ptr, p.colors.ptr,
-                     state[0].ptr, state[1].ptr, background, image_grad.ptr, first.ptr, second.ptr, 1e-4, c1, c2,
+                     state[0].ptr, state[1].ptr, background, image_grad.ptr, moments.ptr, 1e-4, c1, c2,
                      stats.ptr, abs_stats.ptr)
         context.synchronize()
         if step:

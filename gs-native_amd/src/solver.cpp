@@ -15,8 +15,7 @@ Solver::~Solver() {
     DeviceFree(image_gradients_);
     DeviceFree(projection_statistics_);
     DeviceFree(absolute_projection_statistics_);
-    DeviceFree(first_moments_);
-    DeviceFree(second_moments_);
+    DeviceFree(moments_);
 }
 
 bool Solver::Iterate(
@@ -60,9 +59,9 @@ bool Solver::Iterate(
                         reinterpret_cast<const int*>(renderer.GetTileRects()),
                         reinterpret_cast<const float*>(renderer.GetMeans()),
                         reinterpret_cast<const float*>(renderer.GetConicOpacities()),
-                        reinterpret_cast<const float*>(renderer.GetColors()), image, image_gradients_, first_moments_,
-                        second_moments_, position_learning_rate, first_correction, second_correction,
-                        projection_statistics_, absolute_projection_statistics_),
+                        reinterpret_cast<const float*>(renderer.GetColors()), image, image_gradients_, moments_,
+                        position_learning_rate, first_correction, second_correction, projection_statistics_,
+                        absolute_projection_statistics_),
         "gsn_gaussian_gradients_adam");
     if (ok) {
         iteration_ = iteration;
@@ -94,8 +93,7 @@ bool Solver::ResetOpacityHistory() {
         return true;
     }
     gsn_context* context = DefaultContext();
-    return context && Check(gsn_zero_rows(context, first_moments_, second_moments_, gaussian_capacity_, kOpacityIndex, 1),
-                          "gsn_zero_rows");
+    return context && Check(gsn_zero_rows(context, moments_, gaussian_capacity_, kOpacityIndex, 1), "gsn_zero_rows");
 }
 
 bool Solver::RemapGaussians(const int* parent_ids, int gaussian_count) {
@@ -105,11 +103,10 @@ bool Solver::RemapGaussians(const int* parent_ids, int gaussian_count) {
     }
     float* statistics = gaussian_count > 0 && gaussian_count <= gaussian_capacity_ ? projection_statistics_ : nullptr;
     float* absolute = gaussian_count > 0 && gaussian_count <= gaussian_capacity_ ? absolute_projection_statistics_ : nullptr;
-    float* first = nullptr;
-    float* second = nullptr;
+    float* moments = nullptr;
     if (gaussian_count > 0) {
         size_t n = static_cast<size_t>(gaussian_count);
-        bool ok = DeviceAllocateCount(&first, n * kMaxParamCount) && DeviceAllocateCount(&second, n * kMaxParamCount);
+        bool ok = DeviceAllocateCount(&moments, gsn_moment_floats(gaussian_count));
         if (ok && !statistics) {
             ok = DeviceAllocateCount(&statistics, n * 2);
         }
@@ -117,8 +114,7 @@ bool Solver::RemapGaussians(const int* parent_ids, int gaussian_count) {
             ok = DeviceAllocateCount(&absolute, n * 2);
         }
         if (ok) {
-            ok = Check(gsn_remap_history(context, parent_ids, gaussian_capacity_, gaussian_count, first_moments_,
-                           second_moments_, first, second),
+            ok = Check(gsn_remap_history(context, parent_ids, gaussian_capacity_, gaussian_count, moments_, moments),
                      "gsn_remap_history") &&
                 Check(gsn_context_synchronize(context), "gsn_context_synchronize");
         }
@@ -129,8 +125,7 @@ bool Solver::RemapGaussians(const int* parent_ids, int gaussian_count) {
             if (absolute != absolute_projection_statistics_) {
                 DeviceFree(absolute);
             }
-            DeviceFree(first);
-            DeviceFree(second);
+            DeviceFree(moments);
             return false;
         }
     }
@@ -140,12 +135,10 @@ bool Solver::RemapGaussians(const int* parent_ids, int gaussian_count) {
     if (absolute != absolute_projection_statistics_) {
         DeviceFree(absolute_projection_statistics_);
     }
-    DeviceFree(first_moments_);
-    DeviceFree(second_moments_);
+    DeviceFree(moments_);
     projection_statistics_ = statistics;
     absolute_projection_statistics_ = absolute;
-    first_moments_ = first;
-    second_moments_ = second;
+    moments_ = moments;
     gaussian_capacity_ = gaussian_count;
     return true;
 }

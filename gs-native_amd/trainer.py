@@ -18,13 +18,14 @@ import sys
 import numpy as np
 
 if __package__:
-    from . import DeviceBuffer, Projection, adam_corrections, call, gaussian_gradients, gaussian_gradients_adam
-    from . import image_gradients, render, _p
+    from . import CapacityBuffer, DeviceBuffer, Moments, Projection, adam_corrections, call, gaussian_gradients
+    from . import gaussian_gradients_adam, image_gradients, render, _p
 else:  # loaded by path
     _pkg = sys.modules["gs_native_amd"]
     DeviceBuffer, Projection, adam_corrections, call = _pkg.DeviceBuffer, _pkg.Projection, _pkg.adam_corrections, _pkg.call
     gaussian_gradients, gaussian_gradients_adam = _pkg.gaussian_gradients, _pkg.gaussian_gradients_adam
-    image_gradients, render, _p = _pkg.image_gradients, _pkg.render, _pkg._p
+    image_gradients, render, _p, Moments = _pkg.image_gradients, _pkg.render, _pkg._p, _pkg.Moments
+    CapacityBuffer = _pkg.CapacityBuffer
 
 
 FEATURES = 16
@@ -76,14 +77,20 @@ class Trainer:
         self.features_enabled = policy is not None or outcome_log is not None
         self.pending = None  # the previous density step's decisions, kept for its outcome labels
         self.count = int(params.shape[1])
-        self.params = DeviceBuffer.from_numpy(np.ascontiguousarray(params, np.float32))
-        self.first = DeviceBuffer.zeros((58, self.count), np.float32)
-        self.second = DeviceBuffer.zeros((58, self.count), np.float32)
-        self.statistics = DeviceBuffer.zeros((2, self.count), np.float32)
-        self.absolute_statistics = DeviceBuffer.zeros((2, self.count), np.float32)
-        self.sums = DeviceBuffer.zeros((self.count,), np.float32)
-        self.absolute_sums = DeviceBuffer.zeros((self.count,), np.float32)
-        self.observations = DeviceBuffer.zeros((self.count,), np.uint32)
+        # Per-Gaussian state lives in capacity allocations (25% headroom) handed out as views of
+        # the current count, so a density step allocates and frees nothing unless the count
+        # outgrows them; parameters and moments are double-buffered (density control reads the
+        # old and writes the new).
+        self._stores = {name: CapacityBuffer(dtype) for name, dtype in (
+            ("params_a", np.float32), ("params_b", np.float32), ("moments_a", np.float32),
+            ("moments_b", np.float32), ("statistics", np.float32), ("absolute_statistics", np.float32),
+            ("sums", np.float32), ("absolute_sums", np.float32), ("observations", np.uint32),
+            ("decisions", np.int32), ("offsets", np.int32), ("history", np.int32))}
+        self._current = "a"
+        self.params = self._stores["params_a"].get((58, self.count))
+        self.params.upload(np.ascontiguousarray(params, np.float32))
+        self.moments = Moments(self.count, self._stores["moments_a"])
+        self._allocate_statistics(self.count)
         self.cameras = cameras
         self.targets = [t if isinstance(t, DeviceBuffer) else DeviceBuffer.from_numpy(np.ascontiguousarray(t, np.uint8))
                         for t in targets]
@@ -108,6 +115,16 @@ class Trainer:
         self.gradients = None
         if self.features_enabled:
             self._reset_feature_state()
+
+    def _allocate_statistics(self, count: int) -> None:
+        n = max(count, 1)
+        # The gradient pass writes both statistics rows of every Gaussian; the accumulators start
+        # at zero after every density step.
+        self.statistics = self._stores["statistics"].get((2, n))
+        self.absolute_statistics = self._stores["absolute_statistics"].get((2, n))
+        self.sums = self._stores["sums"].get((n,), zero=True)
+        self.absolute_sums = self._stores["absolute_sums"].get((n,), zero=True)
+        self.observations = self._stores["observations"].get((n,), zero=True)
 
     def _reset_feature_state(self) -> None:
         self.feature_state = DeviceBuffer((FEATURE_STATE_ROWS, max(self.count, 1)), np.float32)
@@ -136,13 +153,13 @@ class Trainer:
         self.sh_degree = min(3, iteration // 1000)
         lr = self.position_learning_rate(iteration)
         self.projection = render(self.context, self.params, self.count, camera, self.background, self.image,
-                                 self.sh_degree, self.projection if self.projection and self.projection.count == self.count else None)
+                                 self.sh_degree, self.projection)
         image_gradients(self.context, camera, self.image, self.targets[index], self.image_grad)
         self.adam_step += 1
         c1, c2 = adam_corrections(self.adam_step)
         if self.fused:
             gaussian_gradients_adam(self.context, self.projection, self.params, self.sh_degree, self.image,
-                                    self.image_grad, self.first, self.second, lr, c1, c2, self.statistics,
+                                    self.image_grad, self.moments, lr, c1, c2, self.statistics,
                                     self.absolute_statistics)
         else:
             if self.gradients is None or self.gradients.shape != (58, self.count):
@@ -150,7 +167,7 @@ class Trainer:
             gaussian_gradients(self.context, self.projection, self.params, self.sh_degree, self.image,
                                self.image_grad, self.gradients, self.statistics, self.absolute_statistics)
             call("gsn_adam_update", self.context.handle, self.count, self.gradients.ptr, self.params.ptr,
-                 self.first.ptr, self.second.ptr, lr, c1, c2)
+                 self.moments.ptr, lr, c1, c2)
         self.completed = iteration
         records = self.projection.record_count
         if self.completed < 15000:
@@ -173,14 +190,14 @@ class Trainer:
             self.pending = None
         logit = float(np.float32(math.log(np.float32(cap) / (np.float32(1.0) - np.float32(cap)))))
         call("gsn_density_reset_opacity", self.context.handle, self.params.ptr, self.count, self.count, logit)
-        call("gsn_zero_rows", self.context.handle, self.first.ptr, self.second.ptr, self.count, 9, 1)
+        call("gsn_zero_rows", self.context.handle, self.moments.ptr, self.count, 9, 1)
 
     def control_density(self, gradient_threshold=0.0002, percent_dense=0.01, min_opacity=0.005,
                         absolute_threshold=0.0008) -> tuple[int, int]:
         count = self.count
         max_screen = 20.0 if self.completed > 3000 else 0.0
-        decisions = DeviceBuffer((count + 1, 4), np.int32)
-        offsets = DeviceBuffer((count + 1, 4), np.int32)
+        decisions = self._stores["decisions"].get((count + 1, 4))
+        offsets = self._stores["offsets"].get((count + 1, 4))
         totals = (ctypes.c_int32 * 4)()
         split_threshold = float(np.float32(percent_dense) * np.float32(self.scene_extent))
         world = float(np.float32(0.1) * np.float32(self.scene_extent))
@@ -224,23 +241,17 @@ class Trainer:
         if self.max_count is not None and out_count > self.max_count:
             # Capacity cap: no clones or splits this round, pruning only.
             t, out_count = count_changes(float("inf"), float("inf"))
-        params = DeviceBuffer((58, max(out_count, 1)), np.float32)
-        history = DeviceBuffer((max(out_count, 1),), np.int32)
+        spare = "b" if self._current == "a" else "a"
+        params = self._stores[f"params_{spare}"].get((58, max(out_count, 1)))
+        history = self._stores["history"].get((max(out_count, 1),))
         call("gsn_density_write", self.context.handle, self.params.ptr, count, decisions.ptr, offsets.ptr, totals, 0,
              self.density_counter, params.ptr, out_count, history.ptr, None, None)
         self.density_counter += 1
-        first = DeviceBuffer((58, max(out_count, 1)), np.float32)
-        second = DeviceBuffer((58, max(out_count, 1)), np.float32)
-        call("gsn_remap_history", self.context.handle, history.ptr, count, out_count, self.first.ptr, self.second.ptr,
-             first.ptr, second.ptr)
-        self.params, self.first, self.second = params, first, second
+        moments = Moments(out_count, self._stores[f"moments_{spare}"], zero=False)  # remap writes every pair
+        call("gsn_remap_history", self.context.handle, history.ptr, count, out_count, self.moments.ptr, moments.ptr)
+        self.params, self.moments, self._current = params, moments, spare
         self.count = out_count
-        self.statistics = DeviceBuffer.zeros((2, max(out_count, 1)), np.float32)
-        self.absolute_statistics = DeviceBuffer.zeros((2, max(out_count, 1)), np.float32)
-        self.sums = DeviceBuffer.zeros((max(out_count, 1),), np.float32)
-        self.absolute_sums = DeviceBuffer.zeros((max(out_count, 1),), np.float32)
-        self.observations = DeviceBuffer.zeros((max(out_count, 1),), np.uint32)
-        self.projection = None
+        self._allocate_statistics(out_count)
         if self.features_enabled:
             self._reset_feature_state()
         if self.outcome_log is not None:

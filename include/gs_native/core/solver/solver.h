@@ -49,8 +49,7 @@ private:
 
     float* projection_statistics_ = nullptr;
     float* absolute_projection_statistics_ = nullptr;
-    float* first_moments_ = nullptr;
-    float* second_moments_ = nullptr;
+    float* moments_ = nullptr;  // Adam moments, blocked (gsn_moment_floats floats; see gsn_hip.h)
     int gaussian_capacity_ = 0;
 
     uint64_t iteration_ = 0;

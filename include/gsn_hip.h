@@ -60,6 +60,8 @@ typedef struct gsn_context gsn_context;
 int gsn_context_create(gsn_context** context);
 void gsn_context_destroy(gsn_context* context);
 int gsn_context_synchronize(gsn_context* context);
+/* Waits for all work on the current device (every context's stream). */
+int gsn_device_synchronize(void);
 /* Returns the context's hipStream_t. */
 void* gsn_context_stream(gsn_context* context);
 /* Bytes of scratch memory currently held by the context. */
@@ -205,22 +207,31 @@ int gsn_gaussian_gradients(gsn_context* context, const gsn_camera* camera, int c
     const float* colors, const float* image, const float* image_gradients, float* gradients,
     float* screen_gradients, float* statistics, float* absolute_statistics);
 
+/* Adam moments. The reference keeps two [58][N] arrays (Solver's private first_moments_ /
+ * second_moments_, solver.h:60-63); here they are one blocked buffer, the optimiser's own state:
+ * block b holds Gaussians [64b, 64b+64), and inside it each of the 58 rows holds 64 (first,
+ * second) float pairs, so the pair of Gaussian i in row r is at float
+ * ((i / 64) * 58 + r) * 128 + (i % 64) * 2. gsn_moment_floats(N) is its size in floats (N rounded
+ * up to 64). gsn_moments_unpack / gsn_moments_pack convert to and from two [58][N] arrays (no
+ * counterpart in the reference: tests and checkpoints). */
+size_t gsn_moment_floats(int count);
+int gsn_moments_unpack(gsn_context* context, int count, const float* moments, float* first, float* second);
+int gsn_moments_pack(gsn_context* context, int count, const float* first, const float* second, float* moments);
+
 /* Same as gsn_gaussian_gradients followed by gsn_adam_update, fused so that parameter
  * gradients never go to memory. Replaces Solver::Iterate's gradient + update stages
- * (solver.cu:702-720, 837-929). */
+ * (solver.cu:702-720, 837-929). moments: blocked, gsn_moment_floats(count) floats. */
 int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, int count,
     int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
     int record_count, const int* record_offsets, const int* tile_rects, const float* means,
     const float* conic_opacities, const float* colors, const float* image,
-    const float* image_gradients, float* first_moments, float* second_moments,
-    float position_learning_rate, float first_correction, float second_correction,
-    float* statistics, float* absolute_statistics);
+    const float* image_gradients, float* moments, float position_learning_rate,
+    float first_correction, float second_correction, float* statistics, float* absolute_statistics);
 
 /* Dense Adam over all 58*N entries with the reference's per-group learning rates.
  * Replaces UpdateGaussiansKernel (solver.cu:651-689, 901-929). */
 int gsn_adam_update(gsn_context* context, int count, const float* gradients, float* params,
-    float* first_moments, float* second_moments, float position_learning_rate,
-    float first_correction, float second_correction);
+    float* moments, float position_learning_rate, float first_correction, float second_correction);
 
 /* ---- stored-state rasteriser variant (no counterpart in the reference) -------------------
  * For the peak-memory comparison with the recompute form above, which stores nothing per pixel.
@@ -245,20 +256,18 @@ int gsn_gaussian_gradients_adam_stored(gsn_context* context, const gsn_camera* c
     int record_count, const int* record_offsets, const int* tile_rects, const float* means,
     const float* conic_opacities, const float* colors, const float* final_transmittance,
     const int32_t* last_contributor, const float* background, const float* image_gradients,
-    float* first_moments, float* second_moments, float position_learning_rate,
-    float first_correction, float second_correction, float* statistics,
-    float* absolute_statistics);
+    float* moments, float position_learning_rate, float first_correction, float second_correction,
+    float* statistics, float* absolute_statistics);
 
-/* Gathers Adam moments by parent id (parent < 0 or >= previous_count -> zero).
- * parent_ids may be NULL (all zero). Replaces RemapGaussianHistoryKernel (solver.cu:625-649). */
+/* Gathers Adam moments (blocked buffers, distinct) by parent id (parent < 0 or >= previous_count
+ * -> zero). parent_ids may be NULL (all zero). Replaces RemapGaussianHistoryKernel
+ * (solver.cu:625-649). */
 int gsn_remap_history(gsn_context* context, const int* parent_ids, int previous_count,
-    int count, const float* previous_first, const float* previous_second, float* first,
-    float* second);
+    int count, const float* previous_moments, float* moments);
 
-/* Zeroes rows [row, row+rows) of two [58][stride] moment buffers (opacity history reset,
+/* Zeroes both moments of rows [row, row+rows) of all `count` Gaussians (opacity history reset,
  * solver.cu:722-733). */
-int gsn_zero_rows(gsn_context* context, float* first, float* second, int stride, int row,
-    int rows);
+int gsn_zero_rows(gsn_context* context, float* moments, int count, int row, int rows);
 
 /* ---- density control ------------------------------------------------------------------- */
 

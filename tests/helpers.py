@@ -90,15 +90,16 @@ def solver_step(gsn, context, params: np.ndarray, cam, target: np.ndarray, sh_de
     target_dev = gsn.DeviceBuffer.from_numpy(np.ascontiguousarray(target, np.uint8))
     image_grad = gsn.DeviceBuffer((cam.height, cam.width, 3), np.float32)
     gsn.image_gradients(context, cam, image, target_dev, image_grad)
-    first_dev = gsn.DeviceBuffer.from_numpy(np.zeros((58, n), np.float32) if first is None else first)
-    second_dev = gsn.DeviceBuffer.from_numpy(np.zeros((58, n), np.float32) if second is None else second)
+    zeros = np.zeros((58, n), np.float32)
+    moments = gsn.Moments.from_numpy(zeros if first is None else first, zeros if second is None else second)
     stats = gsn.DeviceBuffer.zeros((2, n), np.float32)
     abs_stats = gsn.DeviceBuffer.zeros((2, n), np.float32)
     c1, c2 = gsn.adam_corrections(step)
-    gsn.gaussian_gradients_adam(context, projection, params_dev, sh_degree, image, image_grad, first_dev, second_dev,
+    gsn.gaussian_gradients_adam(context, projection, params_dev, sh_degree, image, image_grad, moments,
                                 position_lr, c1, c2, stats, abs_stats)
     context.synchronize()
-    return {"params": params_dev.numpy(), "first": first_dev.numpy(), "second": second_dev.numpy(),
+    first_out, second_out = moments.numpy()
+    return {"params": params_dev.numpy(), "first": first_out, "second": second_out,
             "statistics": stats.numpy(), "absolute_statistics": abs_stats.numpy(), "image": image.numpy()}
 
 

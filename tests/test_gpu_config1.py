@@ -46,12 +46,12 @@ def test_density_step_bit_exact_at_config1(gsn, context, oracle, config1):
             "params": trainer.params.numpy()[:, :n].copy(), "sums": trainer.sums.numpy()[:n].copy(),
             "absolute_sums": trainer.absolute_sums.numpy()[:n].copy(),
             "observations": trainer.observations.numpy()[:n].copy(),
-            "first": trainer.first.numpy()[:, :n].copy(), "second": trainer.second.numpy()[:, :n].copy(),
+            "moments": trainer.moments.numpy(),
         }
         changes = control(*args, **kwargs)
         m = trainer.count
         captured["after"] = {"count": m, "params": trainer.params.numpy()[:, :m].copy(),
-                             "first": trainer.first.numpy()[:, :m].copy(), "second": trainer.second.numpy()[:, :m].copy()}
+                             "moments": trainer.moments.numpy()}
         return changes
 
     trainer.control_density = capturing_control
@@ -78,9 +78,9 @@ def test_density_step_bit_exact_at_config1(gsn, context, oracle, config1):
     assert totals[1] + totals[3] > 0, "the step should densify something"
     out, history, _, _ = oracle.density_write_f32(before["params"], decisions, offsets, 0, before["counter"])
     np.testing.assert_array_equal(after["params"].view(np.uint32), out.view(np.uint32))
-    first, second = oracle.remap_f32(history, before["first"], before["second"], expected_count)
-    np.testing.assert_array_equal(after["first"].view(np.uint32), first.view(np.uint32))
-    np.testing.assert_array_equal(after["second"].view(np.uint32), second.view(np.uint32))
+    first, second = oracle.remap_f32(history, *before["moments"], expected_count)
+    np.testing.assert_array_equal(after["moments"][0].view(np.uint32), first.view(np.uint32))
+    np.testing.assert_array_equal(after["moments"][1].view(np.uint32), second.view(np.uint32))
 
 
 def test_config1_training_is_deterministic(gsn, context, config1):
@@ -91,7 +91,7 @@ def test_config1_training_is_deterministic(gsn, context, config1):
             trainer.iteration()
         context.synchronize()
         assert trainer.history and trainer.history[0][0] == 600
-        results.append((trainer.count, trainer.params_host().copy(), trainer.first.numpy()[:, :trainer.count].copy()))
+        results.append((trainer.count, trainer.params_host().copy(), trainer.moments.numpy()[0]))
         del trainer
     assert results[0][0] == results[1][0]
     np.testing.assert_array_equal(results[0][1].view(np.uint32), results[1][1].view(np.uint32))

@@ -88,18 +88,19 @@ def test_gradient_pass_deterministic_and_fused_equals_separate(gsn, context, ora
     rng = np.random.default_rng(14)
     first = rng.normal(0, 1e-4, size=params.shape).astype(np.float32)
     second = rng.uniform(0, 1e-8, size=params.shape).astype(np.float32)
-    first_dev, second_dev = gsn.DeviceBuffer.from_numpy(first), gsn.DeviceBuffer.from_numpy(second)
+    moments = gsn.Moments.from_numpy(first, second)
     stats = gsn.DeviceBuffer.zeros((2, COUNT), np.float32)
     abs_stats = gsn.DeviceBuffer.zeros((2, COUNT), np.float32)
     c1, c2 = gsn.adam_corrections(5)
-    gsn.gaussian_gradients_adam(context, projection, device_params, 3, image, image_grad, first_dev, second_dev,
+    gsn.gaussian_gradients_adam(context, projection, device_params, 3, image, image_grad, moments,
                                 1.6e-4, c1, c2, stats, abs_stats)
     context.synchronize()
     expected_params, expected_first, expected_second = oracle.adam_f32(gradients, params, first, second, 1.6e-4, c1,
                                                                        c2)
     np.testing.assert_array_equal(device_params.numpy().view(np.uint32), expected_params.view(np.uint32))
-    np.testing.assert_array_equal(first_dev.numpy().view(np.uint32), expected_first.view(np.uint32))
-    np.testing.assert_array_equal(second_dev.numpy().view(np.uint32), expected_second.view(np.uint32))
+    got_first, got_second = moments.numpy()
+    np.testing.assert_array_equal(got_first.view(np.uint32), expected_first.view(np.uint32))
+    np.testing.assert_array_equal(got_second.view(np.uint32), expected_second.view(np.uint32))
 
 
 def test_stored_state_backward_agrees_at_full_size(gsn, context, full_scene):

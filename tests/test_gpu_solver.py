@@ -104,14 +104,16 @@ def test_adam_bit_exact(gsn, context, oracle):
     params = rng.normal(size=(58, n)).astype(np.float32)
     first = rng.normal(0, 1e-3, size=(58, n)).astype(np.float32)
     second = rng.uniform(0, 1e-6, size=(58, n)).astype(np.float32)
-    p_dev, m_dev, v_dev = (gsn.DeviceBuffer.from_numpy(a) for a in (params, first, second))
-    gsn.adam_update(context, n, gsn.DeviceBuffer.from_numpy(gradients), p_dev, m_dev, v_dev, 1.6e-4, 7)
+    p_dev = gsn.DeviceBuffer.from_numpy(params)
+    moments = gsn.Moments.from_numpy(first, second)
+    gsn.adam_update(context, n, gsn.DeviceBuffer.from_numpy(gradients), p_dev, moments, 1.6e-4, 7)
     context.synchronize()
     c1, c2 = gsn.adam_corrections(7)
     ep, em, ev = oracle.adam_f32(gradients, params, first, second, 1.6e-4, c1, c2)
+    got_first, got_second = moments.numpy()
     np.testing.assert_array_equal(p_dev.numpy().view(np.uint32), ep.view(np.uint32))
-    np.testing.assert_array_equal(m_dev.numpy().view(np.uint32), em.view(np.uint32))
-    np.testing.assert_array_equal(v_dev.numpy().view(np.uint32), ev.view(np.uint32))
+    np.testing.assert_array_equal(got_first.view(np.uint32), em.view(np.uint32))
+    np.testing.assert_array_equal(got_second.view(np.uint32), ev.view(np.uint32))
 
 
 @pytest.mark.parametrize("sh_degree", [0, 3])
@@ -160,18 +162,16 @@ def test_remap_history_bit_exact(gsn, context, oracle):
     first = rng.normal(size=(58, previous)).astype(np.float32)
     second = rng.normal(size=(58, previous)).astype(np.float32)
     parents = rng.integers(-3, previous + 3, size=count).astype(np.int32)
-    out_first = gsn.DeviceBuffer((58, count), np.float32)
-    out_second = gsn.DeviceBuffer((58, count), np.float32)
+    out = gsn.Moments(count)
     # Keep every input alive until the stream has finished with it.
     parents_dev = gsn.DeviceBuffer.from_numpy(parents)
-    first_dev = gsn.DeviceBuffer.from_numpy(first)
-    second_dev = gsn.DeviceBuffer.from_numpy(second)
-    gsn.call("gsn_remap_history", context.handle, parents_dev.ptr, previous, count, first_dev.ptr, second_dev.ptr,
-             out_first.ptr, out_second.ptr)
+    previous_moments = gsn.Moments.from_numpy(first, second)
+    gsn.call("gsn_remap_history", context.handle, parents_dev.ptr, previous, count, previous_moments.ptr, out.ptr)
     context.synchronize()
     ef, es = oracle.remap_f32(parents, first, second, count)
-    np.testing.assert_array_equal(out_first.numpy(), ef)
-    np.testing.assert_array_equal(out_second.numpy(), es)
+    got_first, got_second = out.numpy()
+    np.testing.assert_array_equal(got_first, ef)
+    np.testing.assert_array_equal(got_second, es)
 
 
 # ---- reference solver cases (test/solver_test.cu) -------------------------------------------
@@ -305,15 +305,13 @@ def test_reference_case_parent_permutation(gsn, context, scenes):
     second = solver_step(gsn, context, params, cam, target_a, 1)
     swapped = second["params"][:, ::-1].copy()
     parents = gsn.DeviceBuffer.from_numpy(np.array([1, 0], np.int32))
-    m = gsn.DeviceBuffer((58, 2), np.float32)
-    v = gsn.DeviceBuffer((58, 2), np.float32)
-    previous_first = gsn.DeviceBuffer.from_numpy(second["first"])
-    previous_second = gsn.DeviceBuffer.from_numpy(second["second"])
-    gsn.call("gsn_remap_history", context.handle, parents.ptr, 2, 2, previous_first.ptr, previous_second.ptr, m.ptr,
-             v.ptr)
+    remapped = gsn.Moments(2)
+    previous = gsn.Moments.from_numpy(second["first"], second["second"])
+    gsn.call("gsn_remap_history", context.handle, parents.ptr, 2, 2, previous.ptr, remapped.ptr)
     context.synchronize()
+    m, v = remapped.numpy()
     a = solver_step(gsn, context, first["params"], cam, target_b, 1, first=first["first"], second=first["second"], step=2)
-    b = solver_step(gsn, context, swapped, cam, target_b, 1, first=m.numpy(), second=v.numpy(), step=2)
+    b = solver_step(gsn, context, swapped, cam, target_b, 1, first=m, second=v, step=2)
     np.testing.assert_allclose(a["params"], b["params"][:, ::-1], atol=2e-6)
 
 
