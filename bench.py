@@ -442,16 +442,26 @@ def main() -> int:
         context.synchronize()
         first_reset = reset_trainer.completed + 1
         records = 0
+        reset_cameras = []
         begin = time.perf_counter()
         for _ in range(k_steps):
             records += reset_trainer.iteration()
+            reset_cameras.append(reset_trainer.last_camera)
         context.synchronize()
         reset_elapsed = time.perf_counter() - begin
+        context.count_raster_pairs(True)
+        context.raster_pairs()
+        for index in reset_cameras:
+            gsn.render(context, reset_trainer.params, reset_trainer.count, cameras[index], reset_trainer.background,
+                       reset_trainer.image, reset_trainer.sh_degree)
+        reset_pairs = context.raster_pairs() / len(reset_cameras)
+        context.count_raster_pairs(False)
         result["after_opacity_cap"] = {
             "value": k_steps / reset_elapsed,
             "ms_per_step": 1000.0 * reset_elapsed / k_steps,
             "iterations": f"{first_reset}-{reset_trainer.completed}",
             "mean_records": records / k_steps,
+            "evaluated_pairs_per_iteration": reset_pairs,
             "note": f"same state entered at iteration {RESET_ITERATION - 1}: the schedule's 0.01 opacity cap is "
                     f"applied after iteration {RESET_ITERATION}, then {args.warmup} warm-up and {k_steps} timed "
                     f"iterations",

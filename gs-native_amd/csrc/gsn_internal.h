@@ -40,6 +40,7 @@ enum ScratchSlot {
     kSlotDepthRects,
     kSlotGaussianOrder,
     kSlotQuantizedError,
+    kSlotKnnOrder,
     kSlotCount
 };
 

@@ -295,7 +295,7 @@ extern "C" int gsn_knn3(gsn_context* context, const float* points, int n, int32_
     float* bounds = static_cast<float*>(scratch(context, kSlotReduce, static_cast<size_t>(bound_blocks) * 6 * 4));
     uint32_t* keys = static_cast<uint32_t*>(scratch(context, kSlotKnnKeys, count * 4));
     uint32_t* ids = static_cast<uint32_t*>(scratch(context, kSlotKnnIds, count * 4));
-    uint32_t* order = static_cast<uint32_t*>(scratch(context, kSlotDepthSortedIds, count * 4));
+    uint32_t* order = static_cast<uint32_t*>(scratch(context, kSlotKnnOrder, count * 4));
     float4* sorted = static_cast<float4*>(scratch(context, kSlotKnnPoints, count * sizeof(float4)));
     Box* nodes = static_cast<Box*>(scratch(context, kSlotKnnNodes, static_cast<size_t>(2 * leaves) * sizeof(Box)));
     if (!bounds || !keys || !ids || !order || !sorted || !nodes) {
