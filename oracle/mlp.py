@@ -94,8 +94,11 @@ def forward(features: np.ndarray, params: np.ndarray, quantize: bool = True) -> 
     return {"x": x, "w1": w1, "w2": w2, "w3": w3, "z1": z1, "a1": a1, "z2": z2, "a2": a2, "logits": z3}
 
 
-def loss_and_gradients(features, labels, params, sample_weights=None, quantize: bool = True):
-    """Weighted mean softmax cross-entropy (weights w_i / n) and its parameter gradients."""
+def loss_and_gradients(features, labels, params, sample_weights=None, quantize: bool = True,
+                       magnitudes: bool = False):
+    """Weighted mean softmax cross-entropy (weights w_i / n) and its parameter gradients. With
+    magnitudes=True also returns, per parameter, the sum of the absolute values of the per-sample
+    terms that make up its gradient (the scale of its rounding error in any summation order)."""
     features = np.asarray(features, np.float32)
     n = features.shape[0]
     labels = np.asarray(labels, np.int64)
@@ -127,7 +130,13 @@ def loss_and_gradients(features, labels, params, sample_weights=None, quantize: 
     dw1 = dz1.T @ f["x"]
     db1 = dz1.sum(axis=0)
     gradients = np.concatenate([dw1.reshape(-1), db1, dw2.reshape(-1), db2, dw3.reshape(-1), db3])
-    return loss, gradients
+    if not magnitudes:
+        return loss, gradients
+    a1, a2, x = np.abs(f["a1"]), np.abs(f["a2"]), np.abs(f["x"])
+    m3, m2, m1 = np.abs(dz3), np.abs(dz2), np.abs(dz1)
+    scale = np.concatenate([(m1.T @ x).reshape(-1), m1.sum(axis=0), (m2.T @ a1).reshape(-1), m2.sum(axis=0),
+                            (m3.T @ a2).reshape(-1), m3.sum(axis=0)])
+    return loss, gradients, scale
 
 
 def adam_f32(params, first, second, gradients, learning_rate, step):
