@@ -120,14 +120,19 @@ def sphere_cameras(count: int, radius: float, seed: int, hemisphere: bool = Fals
     return poses
 
 
-def blob_scene(count: int, seed: int, blobs: int = 64, sh_degree: int = 3, colour_noise: float = 0.05) -> np.ndarray:
+def blob_scene(count: int, seed: int, blobs: int = 64, sh_degree: int = 3, colour_noise: float = 0.05,
+               texture: float = 0.0, scale: float = 1.0) -> np.ndarray:
     """Gaussians sampled from a seeded mixture of anisotropic blobs inside the unit ball (the
     large-scale workload): positions from the mixture, log-scales set later from 3-NN distances,
     opacity logit N(0, 1), SH DC from per-blob colours plus N(0, colour_noise) per Gaussian (the
-    scene's fine texture), higher bands N(0, 0.05)."""
+    scene's fine texture), higher bands N(0, 0.05). texture > 0 adds a seeded colour field of that
+    amplitude, a sum of plane waves with wavelengths 0.03-0.2 (structure between the spacing of a
+    3M-Gaussian scene and of a 200K-point one); it depends on the seed and the position only, so
+    sparse points and a dense scene of the same seed share it. scale multiplies the blob centres
+    and axes (the same draws)."""
     rng = np.random.default_rng(seed)
-    centres = rng.uniform(-1.5, 1.5, size=(blobs, 3))
-    axes = rng.uniform(0.05, 0.4, size=(blobs, 3))
+    centres = rng.uniform(-1.5, 1.5, size=(blobs, 3)) * scale
+    axes = rng.uniform(0.05, 0.4, size=(blobs, 3)) * scale
     rotations = [axis_angle_matrix(rng.normal(size=3), rng.uniform(0, math.pi)) for _ in range(blobs)]
     colours = rng.uniform(0.1, 0.9, size=(blobs, 3))
     labels = rng.integers(0, blobs, size=count)
@@ -144,15 +149,39 @@ def blob_scene(count: int, seed: int, blobs: int = 64, sh_degree: int = 3, colou
         p[10 + 16 * c] = (rgb[:, c] - 0.5) / SH_C0
         if sh_degree > 0:
             p[11 + 16 * c: 26 + 16 * c] = rng.normal(0.0, 0.05, size=(15, count))
+    if texture > 0.0:
+        rgb = np.clip(colours[labels] + texture * texture_field(positions / scale, seed) +
+                      rng.normal(0.0, colour_noise, size=(count, 3)), 0.0, 1.0)
+        for c in range(3):
+            p[10 + 16 * c] = (rgb[:, c] - 0.5) / SH_C0
     return p
 
 
-def shell_plane_scene(count: int, seed: int, sh_degree: int = 3, plane_fraction: float = 0.3) -> np.ndarray:
+def texture_field(positions: np.ndarray, seed: int, waves: int = 12) -> np.ndarray:
+    """[count, 3] colour offsets in about [-1, 1]: `waves` plane waves with random directions,
+    log-uniform wavelengths in [0.03, 0.2], phases and per-channel amplitudes, from the seed."""
+    rng = np.random.default_rng([seed, 7])
+    directions = rng.normal(size=(waves, 3))
+    directions /= np.linalg.norm(directions, axis=1, keepdims=True)
+    wavelengths = np.exp(rng.uniform(np.log(0.03), np.log(0.2), size=waves))
+    phases = rng.uniform(0.0, 2.0 * math.pi, size=waves)
+    amplitudes = rng.normal(size=(waves, 3)) / math.sqrt(waves)
+    field = np.zeros((positions.shape[0], 3))
+    for j in range(waves):
+        wave = np.sin(2.0 * math.pi * (positions @ directions[j]) / wavelengths[j] + phases[j])
+        field += wave[:, None] * amplitudes[j][None, :]
+    return field
+
+
+def shell_plane_scene(count: int, seed: int, sh_degree: int = 3, plane_fraction: float = 0.3,
+                      texture: float = 0.0, colour_noise: float = 0.05) -> np.ndarray:
     """Gaussians on a radius-2 spherical shell (0.05 thick) plus a ground plane at y = 2 (y points
     down in the camera convention of sphere_cameras(hemisphere=True), so the cameras look at the
     shell from above the plane), x and z in [-4, 4]: the 500,000-Gaussian GPU configuration.
     Log-scales are set later from 3-NN distances; random rotations, opacity logit N(0, 1), SH DC
-    from uniform colours, higher bands N(0, 0.05)."""
+    from uniform colours, higher bands N(0, 0.05). texture > 0 replaces the uniform colours by
+    0.5 + texture * texture_field(position) + N(0, colour_noise): a surface pattern shared by every
+    scene of the same seed whatever its count."""
     rng = np.random.default_rng(seed)
     planar = int(round(count * plane_fraction))
     shell = count - planar
@@ -172,6 +201,11 @@ def shell_plane_scene(count: int, seed: int, sh_degree: int = 3, plane_fraction:
         p[10 + 16 * c] = (rgb[:, c] - 0.5) / SH_C0
         if sh_degree > 0:
             p[11 + 16 * c: 26 + 16 * c] = rng.normal(0.0, 0.05, size=(15, count))
+    if texture > 0.0:
+        rgb = np.clip(0.5 + texture * texture_field(positions, seed) + rng.normal(0.0, colour_noise, size=(count, 3)),
+                      0.0, 1.0)
+        for c in range(3):
+            p[10 + 16 * c] = (rgb[:, c] - 0.5) / SH_C0
     return p
 
 

@@ -34,9 +34,10 @@ FEATURE_STATE_ROWS = 6
 
 class OutcomeLog:
     """Training samples for the densification proposal: the features of every Gaussian at a
-    density step paired with the outcome label of that step (gsn_density_outcomes), collected on
-    the host. Pruned Gaussians (label -1) are dropped; label-0 samples are kept with probability
-    keep_fraction and weighted 1 / keep_fraction, so the weighted loss stays unbiased."""
+    density step paired with the outcome labels of that step (gsn_density_outcomes, one column
+    per usefulness threshold of the trainer), collected on the host. Pruned Gaussians (label -1)
+    are dropped; samples labelled 0 in every column are kept with probability keep_fraction and
+    weighted 1 / keep_fraction, so the weighted loss of every column stays unbiased."""
 
     def __init__(self, keep_fraction: float = 1.0, seed: int = 0):
         self.keep_fraction = keep_fraction
@@ -46,16 +47,25 @@ class OutcomeLog:
         self.weights: list[np.ndarray] = []
 
     def add(self, features: np.ndarray, labels: np.ndarray) -> None:
-        keep = labels > 0
-        keep |= (labels == 0) & (self.rng.random(labels.shape[0]) < self.keep_fraction)
+        labels = labels.reshape(labels.shape[0], -1)
+        densified = np.any(labels > 0, axis=1)
+        kept = np.all(labels >= 0, axis=1)
+        keep = kept & (densified | (self.rng.random(labels.shape[0]) < self.keep_fraction))
         self.features.append(features[keep])
-        self.labels.append(labels[keep].astype(np.int32))
-        self.weights.append(np.where(labels[keep] == 0, 1.0 / self.keep_fraction, 1.0).astype(np.float32))
+        self.labels.append(labels[keep].astype(np.int8))
+        self.weights.append(np.where(densified[keep], 1.0, 1.0 / self.keep_fraction).astype(np.float32))
 
-    def arrays(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    def arrays(self, column: int = 0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         if not self.features:
             return np.zeros((0, FEATURES), np.float32), np.zeros(0, np.int32), np.zeros(0, np.float32)
-        return np.concatenate(self.features), np.concatenate(self.labels), np.concatenate(self.weights)
+        return (np.concatenate(self.features), np.concatenate([l[:, column] for l in self.labels]).astype(np.int32),
+                np.concatenate(self.weights))
+
+    def samples(self) -> int:
+        return sum(f.shape[0] for f in self.features)
+
+    def represented(self) -> float:
+        return float(sum(w.sum() for w in self.weights))
 
 
 class Trainer:
@@ -68,12 +78,13 @@ class Trainer:
 
     def __init__(self, context, params: np.ndarray, cameras: list, targets: list, scene_extent: float,
                  background=(0.0, 0.0, 0.0), max_count: int | None = None, completed: int = 0, seed: int = 0,
-                 policy=None, outcome_log: OutcomeLog | None = None, useful_opacity: float = 0.02):
+                 policy=None, outcome_log: OutcomeLog | None = None, useful_opacity=0.02):
         self.context = context
         self.policy = policy
         self.policy_fallbacks = 0  # density steps where the growth guard handed the decision back to the rule
         self.outcome_log = outcome_log
-        self.useful_opacity = useful_opacity
+        # One label column per usefulness threshold (a float or a sequence of them).
+        self.useful_opacities = [float(v) for v in np.atleast_1d(useful_opacity)]
         self.features_enabled = policy is not None or outcome_log is not None
         self.pending = None  # the previous density step's decisions, kept for its outcome labels
         self.count = int(params.shape[1])
@@ -268,9 +279,12 @@ class Trainer:
             return
         labels = DeviceBuffer((previous_count,), np.int32)
         totals = (ctypes.c_int32 * 4)(*t)
-        call("gsn_density_outcomes", self.context.handle, decisions.ptr, offsets.ptr, totals, previous_count,
-             self.params.ptr, self.observations.ptr, count, self.useful_opacity, labels.ptr)
-        self.outcome_log.add(previous_features, labels.numpy())
+        columns = []
+        for threshold in self.useful_opacities:
+            call("gsn_density_outcomes", self.context.handle, decisions.ptr, offsets.ptr, totals, previous_count,
+                 self.params.ptr, self.observations.ptr, count, threshold, labels.ptr)
+            columns.append(labels.numpy())
+        self.outcome_log.add(previous_features, np.stack(columns, axis=1))
 
     def params_host(self) -> np.ndarray:
         return self.params.numpy()[:, : self.count]

@@ -141,7 +141,7 @@ def test_trainer_logs_outcomes_and_runs_a_policy(gsn, context, scenes):
     from oracle import mlp
     params, cams, targets = _small_scene(gsn, scenes, context, 20000, 128, 8)
     log = trainer_module.OutcomeLog()
-    trainer = trainer_module.Trainer(context, params, cams, targets, 5.0, outcome_log=log)
+    trainer = trainer_module.Trainer(context, params, cams, targets, 5.0, outcome_log=log, useful_opacity=(0.02, 0.2))
     for _ in range(800):
         trainer.iteration()
     context.synchronize()
@@ -150,6 +150,10 @@ def test_trainer_logs_outcomes_and_runs_a_policy(gsn, context, scenes):
     assert np.all(np.isfinite(features))
     assert set(np.unique(labels)) <= {0, 1, 2}
     assert np.all(weights == 1.0)
+    # a stricter usefulness threshold only turns densify labels into keep labels
+    _, strict, _ = log.arrays(1)
+    assert np.all((strict == 0) | (strict == labels))
+    assert np.sum(strict > 0) <= np.sum(labels > 0)
     # a network that always answers keep: no Gaussian is added at any density step
     parts = {name: np.zeros(shape, np.float32) for name, shape in mlp.SHAPES}
     parts["b3"][:] = [1.0, 0.0, 0.0]
