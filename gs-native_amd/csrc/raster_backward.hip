@@ -102,12 +102,20 @@ __device__ void schedule_units(const Offset* __restrict__ offsets, int stride, i
         atomicAdd(&histogram[bin(u)], 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int running = 0;
-        for (int b = 0; b < kScheduleBins; ++b) {
-            cursor[b] = running;
-            running += histogram[b];
+    // Exclusive scan of the bins by the first wave, two bins per lane.
+    static_assert(kScheduleBins == 128, "the scan below covers 2 x 64 bins");
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const int first = histogram[2 * lane];
+        const int pair = first + histogram[2 * lane + 1];
+        int inclusive = pair;
+#pragma unroll
+        for (int offset = 1; offset < 64; offset <<= 1) {
+            const int below = __shfl_up(inclusive, offset);
+            inclusive += lane >= offset ? below : 0;
         }
+        cursor[2 * lane] = inclusive - pair;
+        cursor[2 * lane + 1] = inclusive - pair + first;
     }
     __syncthreads();
 #pragma unroll
@@ -162,15 +170,15 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
     const int px = x0 + (lane & 15);
     const int py = y0 + (lane >> 4);
     const float fx = static_cast<float>(px);
+    // A pixel that is done (outside the image, or terminated) carries T = 0: every later splat
+    // then finds T (1 - alpha) below the termination threshold and blends nothing.
     float fy[4], T[4], final_color[4][3], grad[4][3], prefix[4][3];
-    bool done[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         int y = py + 4 * b;
         bool inside = px < cam.width && y < cam.height;
         fy[b] = static_cast<float>(y);
-        done[b] = !inside;
-        T[b] = 1.0f;
+        T[b] = inside ? 1.0f : 0.0f;
         int64_t pixel = (static_cast<int64_t>(y) * cam.width + px) * 3;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
         int active = 0;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            if (__ballot(!done[b])) {
+            if (__ballot(T[b] > 0.0f)) {
                 active |= 1 << b;
             }
         }
@@ -263,11 +271,10 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
                     float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbv * dx * dy;
                     float raw_alpha = op * __expf(power);
                     float alpha = fminf(kAlphaMax, raw_alpha);
-                    bool take = !done[b] && !(power > 0.0f) && alpha >= kAlphaMin;
+                    bool take = !(power > 0.0f) && alpha >= kAlphaMin;
                     float next = T[b] * (1.0f - alpha);
                     bool finish = take && next < kTransmittanceMin;
                     bool blend = take && !finish;
-                    done[b] = done[b] || finish;
                     // Uniform branch on "some lane blends"; inside, lanes that do not blend add
                     // exact zeros (selects, not divergent ifs), which keeps the accumulators in
                     // place instead of merging per-lane control paths.
@@ -300,6 +307,7 @@ __global__ __launch_bounds__(64) void raster_backward_kernel(const int* __restri
                         }
                         T[b] = blend ? next : T[b];
                     }
+                    T[b] = finish ? 0.0f : T[b];
                 }
             }
             if (contributed) {
