@@ -126,12 +126,18 @@ __device__ __forceinline__ void ssim_coefficients(const float (&m)[5], bool insi
     }
 }
 
+// Work items are (tile, channel) pairs, channel fastest, and each XCD runs a contiguous range of
+// them (xcd_remap): the three channel workgroups of a tile, which read the same interleaved RGB
+// lines, and neighbouring tiles, which share halo lines, then meet in one XCD's L2.
 __global__ __launch_bounds__(kThreads) void image_gradients_kernel(const float* __restrict__ image,
-    const uint8_t* __restrict__ target, float* __restrict__ gradients, int width, int height, Window win) {
+    const uint8_t* __restrict__ target, float* __restrict__ gradients, int width, int height, int tiles_x,
+    int items, Window win) {
     __shared__ SsimTile s;
-    const int channel = blockIdx.z;
-    const int origin_x = blockIdx.x * kTw;
-    const int origin_y = blockIdx.y * kTh;
+    const int item = xcd_remap(static_cast<int>(blockIdx.x), items);
+    const int channel = item % 3;
+    const int tile = item / 3;
+    const int origin_x = (tile % tiles_x) * kTw;
+    const int origin_y = (tile / tiles_x) * kTh;
     const int t = threadIdx.x;
     // 1. Image and target with a 10-pixel halo (zero outside the image).
     for (int idx = t; idx < kIh * kIw; idx += kThreads) {
@@ -359,10 +365,14 @@ extern "C" int gsn_image_gradients(gsn_context* context, int width, int height, 
     if (!context || width <= 0 || height <= 0 || !image || !target || !image_gradients) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_image_gradients: invalid arguments");
     }
-    dim3 grid(grid_for(width, kTw), grid_for(height, kTh), 3);
+    const int tiles_x = static_cast<int>(grid_for(width, kTw));
+    const int64_t items = static_cast<int64_t>(tiles_x) * grid_for(height, kTh) * 3;
+    if (items > 0x7fffffff) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_image_gradients: image too large");
+    }
     StageScope scope(context, kStageImageGradients);
-    image_gradients_kernel<<<grid, kThreads, 0, context->stream>>>(image, target, image_gradients, width, height,
-        window());
+    image_gradients_kernel<<<static_cast<unsigned>(items), kThreads, 0, context->stream>>>(image, target,
+        image_gradients, width, height, tiles_x, static_cast<int>(items), window());
     return check_launch("image_gradients_kernel");
 }
 
