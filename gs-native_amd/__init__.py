@@ -141,6 +141,7 @@ _PROTOTYPES = {
     "gsn_mlp_train_step": [_P, _P, _P, _P, _I, _P, _P, _P, _P, _F, _I, _P, ctypes.POINTER(ctypes.c_double)],
     "gsn_selftest_wave_sum12": [_P, _P, _P],
     "gsn_selftest_band_mask": [_P, _P, _P, _I, _P],
+    "gsn_selftest_sort_pairs": [_P, _P, _P, _I, _I, _P, _P],
     "gsn_context_set_timing": [_P, _I],
     "gsn_context_set_timing_mask": [_P, ctypes.c_uint],
     "gsn_context_stage_times": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), _I],

@@ -242,3 +242,13 @@ int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int
 }
 
 } // namespace gsn
+
+extern "C" int gsn_selftest_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int count, int bits,
+    uint32_t* keys_out, uint32_t* values_out) {
+    using namespace gsn;
+    if (!context || count < 0 || bits < 1 || bits > 32 ||
+        (count > 0 && (!keys || !values || !keys_out || !values_out || keys_out == keys || values_out == values))) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_selftest_sort_pairs: invalid arguments");
+    }
+    return radix_sort_pairs(context, keys, values, count, bits, keys_out, values_out);
+}

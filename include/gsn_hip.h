@@ -116,6 +116,14 @@ int gsn_selftest_wave_sum12(gsn_context* context, const float* input, float* out
  * reference (its rasteriser tests every pixel of the tile). */
 int gsn_selftest_band_mask(gsn_context* context, const float* splats, const int* origins, int count, int* masks);
 
+/* Diagnostic: the library's stable radix sort of (uint32 key, uint32 value) pairs by key bits
+ * [0, bits) (1 <= bits <= 32), as the depth order, the tile lists and the KNN use it: keys and
+ * values (device, count each) are consumed, the sorted pairs go to keys_out / values_out (device,
+ * distinct from the inputs). Its counterpart in the reference is the CUB
+ * DeviceRadixSort::SortPairs call of renderer.cu:546-563. */
+int gsn_selftest_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int count, int bits,
+    uint32_t* keys_out, uint32_t* values_out);
+
 /* Events on the context stream, for timing. */
 int gsn_event_create(void** event);
 int gsn_event_destroy(void* event);
