@@ -48,3 +48,19 @@ def build_workload(gsn, context, gaussians: int, width: int, height: int, camera
         visible.append(int((counts > 0).sum()))
     del device_params
     return params, cams, targets, extent, visible
+
+
+def initial_from_points(gsn, context, points: np.ndarray) -> np.ndarray:
+    """The reference initialisation from a point cloud (gaussian_splatting.cpp:95-128): positions
+    of `points` (a [58][n] scene of which rows 0-2 and the SH DC rows are used), log-scales from the
+    mean 3-NN squared distance (GPU LBVH), identity rotation, opacity 0.1, SH DC from the point
+    colour, higher bands zero."""
+    count = points.shape[1]
+    init = np.zeros((58, count), np.float32)
+    init[0:3] = points[0:3]
+    _, d2 = gsn.knn3(context, points[0:3].T.copy())
+    init[3:6] = scenes.scales_from_mean_dist2(d2)[None, :]
+    init[9] = np.float32(np.log(0.1 / 0.9))
+    for c in range(3):
+        init[10 + 16 * c] = points[10 + 16 * c]
+    return init
