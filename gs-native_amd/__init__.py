@@ -593,10 +593,14 @@ class ProposalNetwork:
         self.weights = DeviceBuffer((self.weight_count,), np.uint16)
         self.step = 0
         call("gsn_mlp_sync_weights", context.handle, _p(self.params), _p(self.weights))
+        # The network is then ready for use on any context's stream.
+        context.synchronize()
 
-    def forward(self, features: DeviceBuffer, count: int, logits: DeviceBuffer) -> None:
-        call("gsn_mlp_forward", self.context.handle, _p(features), count, _p(self.weights), _p(self.params),
-             _p(logits))
+    def forward(self, features: DeviceBuffer, count: int, logits: DeviceBuffer, context: "Context | None" = None) -> None:
+        """Logits of `count` feature rows, computed on `context`'s stream (the one that produced the
+        features and will read the logits; default: the network's own context)."""
+        stream = (context or self.context).handle
+        call("gsn_mlp_forward", stream, _p(features), count, _p(self.weights), _p(self.params), _p(logits))
 
     def train_step(self, features: DeviceBuffer, labels: DeviceBuffer, count: int, learning_rate: float,
                    sample_weights: DeviceBuffer | None = None, gradients: DeviceBuffer | None = None,

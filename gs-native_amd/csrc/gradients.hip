@@ -533,19 +533,38 @@ __global__ __launch_bounds__(256) void adam_kernel(const float* __restrict__ gra
     store_moments(moments, pair, m, v);
 }
 
+// One thread per new Gaussian: its parent id is read once and its 58 moment pairs are gathered
+// kRemapRows rows at a time (all loads of a group in flight); neighbouring new ids mostly have
+// neighbouring parents, so each row's gathers and stores stay within one or two 512-byte pieces.
+constexpr int kRemapRows = 8;
+
 __global__ __launch_bounds__(256) void remap_history_kernel(const int* parent_ids, const float* previous, float* moments,
     int previous_count, int count) {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= count) {
         return;
     }
-    int row = blockIdx.y;
-    int parent = parent_ids ? parent_ids[id] : -1;
-    float2 pair = make_float2(0.0f, 0.0f);
-    if (parent >= 0 && parent < previous_count) {
-        pair = *reinterpret_cast<const float2*>(previous + moment_index(parent, row));
+    const int parent = parent_ids ? parent_ids[id] : -1;
+    const bool inherited = parent >= 0 && parent < previous_count;
+#pragma unroll
+    for (int base = 0; base < kParamCount; base += kRemapRows) {
+        float2 pairs[kRemapRows];
+#pragma unroll
+        for (int k = 0; k < kRemapRows; ++k) {
+            const int row = base + k;
+            pairs[k] = make_float2(0.0f, 0.0f);
+            if (row < kParamCount && inherited) {
+                pairs[k] = *reinterpret_cast<const float2*>(previous + moment_index(parent, row));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kRemapRows; ++k) {
+            const int row = base + k;
+            if (row < kParamCount) {
+                *reinterpret_cast<float2*>(moments + moment_index(id, row)) = pairs[k];
+            }
+        }
     }
-    *reinterpret_cast<float2*>(moments + moment_index(id, row)) = pair;
 }
 
 // Blocked moments <-> two [58][count] arrays (first and second moments).
@@ -846,9 +865,8 @@ extern "C" int gsn_remap_history(gsn_context* context, const int* parent_ids, in
     if (!moments || (previous_count > 0 && parent_ids && !previous_moments) || previous_moments == moments) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_remap_history: null or aliased moment buffer");
     }
-    dim3 grid(grid_for(count, 256), kParamCount);
     StageScope scope(context, kStageDensity);
-    remap_history_kernel<<<grid, 256, 0, context->stream>>>(parent_ids, previous_moments, moments,
+    remap_history_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(parent_ids, previous_moments, moments,
         previous_moments ? previous_count : 0, count);
     return check_launch("remap_history_kernel");
 }

@@ -53,6 +53,7 @@ def _load_package():
 gsn = _load_package()
 import gs_native_amd.scenes as scenes  # noqa: E402
 import gs_native_amd.trainer as trainer_module  # noqa: E402
+import gs_native_amd.workloads as workloads  # noqa: E402
 
 PARAMETERS = 5443
 
@@ -77,15 +78,7 @@ class Scene:
         truth = generate(truth_count, 3)
         _, d2 = gsn.knn3(context, truth[0:3].T.copy())
         truth[3:6] = scenes.scales_from_mean_dist2(d2)[None, :]
-        points = generate(init_count, 0)
-        init = np.zeros((58, init_count), np.float32)
-        init[0:3] = points[0:3]
-        _, d2 = gsn.knn3(context, points[0:3].T.copy())
-        init[3:6] = scenes.scales_from_mean_dist2(d2)[None, :]
-        init[9] = np.float32(math.log(0.1 / 0.9))
-        for c in range(3):
-            init[10 + 16 * c] = points[10 + 16 * c]
-        self.init = init
+        self.init = workloads.initial_from_points(gsn, context, generate(init_count, 0))
         focal = scenes.fov_focal(width, 60.0)
         make = lambda poses: [gsn.Camera.make(width, height, focal, focal, 0.5 * (width - 1), 0.5 * (height - 1), R, t)
                               for R, t in poses]
@@ -398,6 +391,9 @@ def main() -> int:
     parser.add_argument("--skip-eval", action="store_true", help="collect, train and select only")
     parser.add_argument("--quick", action="store_true", help="small sizes for a functional check")
     parser.add_argument("--out", type=str, default="")
+    parser.add_argument("--record", type=str, default="",
+                        help="write the record the at-scale GPU test checks against: the configuration and the "
+                             "first evaluation seed's outcome (needs the evaluation)")
     args = parser.parse_args()
     if args.quick:
         args.train_seeds, args.eval_seeds = 2, 1
@@ -428,6 +424,13 @@ def main() -> int:
         summary["accepted_all"] = all(e["accepted"] for e in summary["evaluations"])
     if args.out:
         pathlib.Path(args.out).write_text(json.dumps(summary, indent=1))
+    if args.record and "evaluations" in summary:
+        first = summary["evaluations"][0]
+        record = {"config": summary["config"], "held_out": {
+            "seed": first["seed"], "accepted": first["accepted"], "psnr_delta_db": first["psnr_delta_db"],
+            "rule_psnr": first["rule"]["final_psnr"], "learned_psnr": first["learned"]["final_psnr"],
+            "rule_count": first["rule"]["final_count"], "learned_count": first["learned"]["final_count"]}}
+        pathlib.Path(args.record).write_text(json.dumps(record, indent=1))
     print(json.dumps(summary), flush=True)
     return 0
 

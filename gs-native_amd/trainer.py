@@ -224,7 +224,7 @@ class Trainer:
         logits = None
         if self.policy is not None and count > 0:
             logits = DeviceBuffer((count, 3), np.float32)
-            self.policy.forward(features, count, logits)
+            self.policy.forward(features, count, logits, context=self.context)
 
         def count_changes(threshold, absolute, use_policy=True):
             if use_policy and logits is not None and np.isfinite(threshold):

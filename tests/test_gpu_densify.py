@@ -175,3 +175,27 @@ def test_trainer_logs_outcomes_and_runs_a_policy(gsn, context, scenes):
     assert split_all.policy_fallbacks == len(split_all.history) > 0
     assert split_all.history == trainer.history
     assert split_all.count == trainer.count
+
+
+def test_policy_runs_on_the_trainers_stream(gsn, context, scenes):
+    """A network made on one context drives trainers on contexts of their own: its forward pass
+    runs on the trainer's stream (after the features, before the decisions read the logits), so
+    two such runs agree bit for bit."""
+    import gs_native_amd.trainer as trainer_module
+    from oracle import mlp
+    params, cams, targets = _small_scene(gsn, scenes, context, 20000, 128, 8)
+    parts = mlp.unpack(mlp.init_params(3))
+    parts["b3"][:] = [0.0, 0.2, -0.2]
+    network = gsn.ProposalNetwork(context, mlp.pack(parts))
+    runs = []
+    for _ in range(2):
+        own = gsn.Context()
+        trainer = trainer_module.Trainer(own, params, cams, targets, 5.0, policy=network)
+        for _ in range(800):
+            trainer.iteration()
+        own.synchronize()
+        runs.append((trainer.history, trainer.params_host().copy()))
+        del trainer
+        own.close()
+    assert runs[0][0] == runs[1][0]
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
