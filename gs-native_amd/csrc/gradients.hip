@@ -6,7 +6,7 @@
 //   CollectProjectionStatisticsKernel 468-490, MapGaussianGradientsKernel<D> 492-623
 //   (ComputeEquationRotation 185-221, ComputeEquationSh<D> 223-323),
 //   UpdateGaussiansKernel 651-689, RemapGaussianHistoryKernel 625-649.
-#include "gsn_internal.h"
+#include "raster_common.h"
 
 #include <cmath>
 
@@ -287,6 +287,9 @@ constexpr int kAdamGroup = 16;
 
 constexpr int kGradientBlock = kGradientGroup;
 constexpr int kChunks = 2;  // 64-record chunks per wave pass
+// A Gaussian owning at least this many stamped records of a 64-record chunk has them summed by the
+// whole wave (wave_sum12) instead of one by one by its own lane.
+constexpr int kWideOwner = 16;
 
 // Projection statistics of one Gaussian (reference CollectProjectionStatisticsKernel,
 // solver.cu:468-490): pixel-space norms of the mean gradient and of its absolute counterpart, and
@@ -314,7 +317,9 @@ __device__ __forceinline__ void write_statistics(const Camera& cam, const float 
 // kChunks x 64 records at a time, one record per lane and chunk: the flag bytes are read
 // together, the stamped partials are loaded (coalesced, neighbouring lanes hold neighbouring
 // records) into the wave's LDS slots, and each lane then adds the stamped records of its own
-// Gaussian from LDS in record order, so results are identical run to run. Work per wave
+// Gaussian from LDS in record order; a Gaussian owning kWideOwner or more of a chunk's stamped
+// records instead adds the chunk's wave-wide sum of them. Every order is fixed, so results are
+// identical run to run. Work per wave
 // follows the length of its run, not the largest rectangle in it, and each pass costs two
 // dependent global accesses (flags, then partials) for kChunks x 64 records.
 __device__ __forceinline__ void reduce_records(const float* __restrict__ partials,
@@ -391,6 +396,46 @@ __device__ __forceinline__ void reduce_records(const float* __restrict__ partial
             uint64_t mine = ballots[c];
             mine &= high >= 64 ? ~0ull : ((1ull << high) - 1);
             mine &= low >= 64 ? 0ull : ~((1ull << low) - 1);
+            // Wide owners: each lane takes its own record position if the owner stamped it there,
+            // the wave sums the twelve fields and the owner adds the chunk's total. A single new
+            // Gaussian can own thousands of records (a screen-filling split), which one lane would
+            // otherwise add one LDS round trip at a time while its wave holds the pass open.
+            const uint64_t wide = __ballot(__popcll(mine) >= kWideOwner);
+            if (wide) {
+                const int own = 3 * (c * 64 + lane);
+                const float4 a = slots[own], b = slots[own + 1], cc = slots[own + 2];
+                uint64_t rest = wide;
+                while (rest) {
+                    const int j = __builtin_ctzll(rest);
+                    rest &= rest - 1;
+                    const uint32_t owned_lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(mine), j);
+                    const uint32_t owned_hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(mine >> 32), j);
+                    const bool take = (((static_cast<uint64_t>(owned_hi) << 32) | owned_lo) >> lane) & 1;
+                    const float v[12] = {take ? a.x : 0.0f, take ? a.y : 0.0f, take ? a.z : 0.0f,
+                        take ? a.w : 0.0f, take ? b.x : 0.0f, take ? b.y : 0.0f, take ? b.z : 0.0f,
+                        take ? b.w : 0.0f, take ? cc.x : 0.0f, take ? cc.y : 0.0f, take ? cc.z : 0.0f, 0.0f};
+                    float out[3];
+                    wave_sum12(v, out);
+                    float total[12];
+#pragma unroll
+                    for (int row = 0; row < 4; ++row) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            total[3 * row + k] = __uint_as_float(
+                                __builtin_amdgcn_readlane(__float_as_uint(out[k]), 16 * row));
+                        }
+                    }
+                    if (lane == j) {
+#pragma unroll
+                        for (int k = 0; k < 11; ++k) {
+                            in[k] += total[k];
+                        }
+                    }
+                }
+                if (__popcll(mine) >= kWideOwner) {
+                    mine = 0;
+                }
+            }
             while (mine) {
                 const int k = __builtin_ctzll(mine);
                 mine &= mine - 1;

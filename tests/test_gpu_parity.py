@@ -206,6 +206,27 @@ def test_config2_views_match_oracle(gsn, context, oracle, scenes, view):
     raster_and_chain_parity(gsn, context, oracle, params, cam, target, 3, 0.97, f"config2 view {view}")
 
 
+def test_screen_filling_splats_match_oracle(gsn, context, oracle, scenes):
+    """600 large Gaussians (config-0 distributions with log-scales raised by 2.5) at 256x256, SH
+    degree 3: most own one to two hundred records, so nearly every 64-record chunk of the
+    per-Gaussian pass is owned by few Gaussians and summed by the wave-wide reduction rather than
+    lane by lane. Forward, backward and chain rule against FP64. The large overlapping splats make
+    the screen-space sums cancel, so fewer elements are within 1e-5 of |fp64| alone (about 75% for
+    the position and conic rows, the same share as a lane-by-lane sum of the same records gives;
+    the wave-wide sums have the lower normwise error); every element stays within
+    1e-5 (|fp64| + magnitude) and the normwise error within 1e-6."""
+    params = scenes.config0_params(seed=31, count=600, sh_degree=3)
+    params[3:6] += np.float32(2.5)
+    truth = scenes.perturb(params, 32, 0.01)
+    cam = config0_camera(gsn)
+    target = np.clip(np.rint(oracle.render_reference(truth, cam, (0.0, 0.0, 0.0), 3) * 255.0), 0, 255).astype(np.uint8)
+    stages = raster_and_chain_parity(gsn, context, oracle, params, cam, target, 3, 0.70, "screen-filling")
+    owned = np.diff(stages["run"].projection.record_offsets.numpy())
+    print(f"[screen-filling] records per Gaussian: median {np.median(owned):.0f}, max {owned.max()}, "
+          f"share with 16 or more {np.mean(owned >= 16):.3f}", flush=True)
+    assert np.mean(owned >= 64) >= 0.5
+
+
 def test_needle_splats_take_the_fp64_conic_and_match_fp64(gsn, context, oracle):
     """Thin splats seen side-on, diagonal on screen: their 2D covariance has det < 1e-3 a c, so
     the device forms det and the conic from exact FP64 products (the one FP64 path on the GPU).
