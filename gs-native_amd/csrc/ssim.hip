@@ -3,7 +3,7 @@
 //
 // L = 0.8 * mean|I - T| + 0.2 * (1 - mean SSIM) over 3*W*H values; SSIM uses the separable
 // 11-tap Gaussian window (sigma 1.5, normalised) with zero padding, C1 = 1e-4, C2 = 9e-4.
-// One workgroup computes a 32x16 output tile of one channel: a 10-pixel halo is staged in LDS,
+// One workgroup computes a 32x32 output tile of one channel: a 10-pixel halo is staged in LDS,
 // the five window moments are filtered horizontally then vertically, the per-position SSIM
 // derivative coefficients (a, b, c) are formed, and the adjoint filter is applied (again
 // separably) to give dL/dI = (0.8 sign(I - T) - 0.2 (Wa + T Wb + 2 I Wc)) / (3 W H).
@@ -16,12 +16,12 @@ namespace gsn {
 namespace {
 
 constexpr int kTw = 32;
-constexpr int kTh = 16;
+constexpr int kTh = 32;
 constexpr int kIw = kTw + 20;  // input columns (10-pixel halo each side)
 constexpr int kIh = kTh + 20;  // input rows
 constexpr int kMw = kTw + 10;  // moment columns (output +- 5)
 constexpr int kMh = kTh + 10;  // moment rows after the vertical pass
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 
 struct Window {
     float w[11];
@@ -104,9 +104,9 @@ __device__ __forceinline__ void ssim_moments(SsimTile& s, const Window& win, con
 // from LDS once (11-tap windows overlap by 10), which cuts LDS reads 3-4x against one thread per
 // output; every output still sums its taps in the same order, so results are unchanged.
 constexpr int kRunH = 6;   // horizontal moments: outputs per thread (kMw = 42 = 7 runs)
-constexpr int kRunV = 7;   // vertical moments: rows per thread (kMh = 26: runs of 7, 7, 7, 5)
+constexpr int kRunV = 4;   // vertical moments: rows per thread (kMh = 42: ten runs of 4, one of 2)
 constexpr int kRunA = 4;   // horizontal adjoint: outputs per thread (kTw = 32 = 8 runs)
-constexpr int kRunO = 2;   // vertical adjoint: rows per thread (kTh = 16 = 8 runs)
+constexpr int kRunO = 2;   // vertical adjoint: rows per thread (kTh = 32 = 16 runs)
 static_assert(kMw % kRunH == 0 && kTw % kRunA == 0 && kTh % kRunO == 0, "runs must tile the rows");
 
 __device__ __forceinline__ void ssim_coefficients(const float (&m)[5], bool inside, float& a, float& b, float& c) {
@@ -153,11 +153,15 @@ __global__ __launch_bounds__(kThreads) void image_gradients_kernel(const float* 
     for (int item = t; item < kIh * (kMw / kRunH); item += kThreads) {
         const int iy = item / (kMw / kRunH);
         const int column = (item - iy * (kMw / kRunH)) * kRunH;
-        float a[kRunH + 10], b[kRunH + 10];
+        // The products of each input value are formed once and shared by the 11 taps that read it.
+        float a[kRunH + 10], b[kRunH + 10], aa[kRunH + 10], bb[kRunH + 10], ab[kRunH + 10];
 #pragma unroll
         for (int j = 0; j < kRunH + 10; ++j) {
             a[j] = s.in[iy * kIw + column + j];
             b[j] = s.in[kIh * kIw + iy * kIw + column + j];
+            aa[j] = a[j] * a[j];
+            bb[j] = b[j] * b[j];
+            ab[j] = a[j] * b[j];
         }
 #pragma unroll
         for (int g = 0; g < kRunH; ++g) {
@@ -165,13 +169,11 @@ __global__ __launch_bounds__(kThreads) void image_gradients_kernel(const float* 
 #pragma unroll
             for (int dx = 0; dx < 11; ++dx) {
                 float w = win.w[dx];
-                float av = a[g + dx];
-                float bv = b[g + dx];
-                m0 += w * av;
-                m1 += w * bv;
-                m2 += w * av * av;
-                m3 += w * bv * bv;
-                m4 += w * av * bv;
+                m0 += w * a[g + dx];
+                m1 += w * b[g + dx];
+                m2 += w * aa[g + dx];
+                m3 += w * bb[g + dx];
+                m4 += w * ab[g + dx];
             }
             const int idx = iy * kMw + column + g;
             s.h[0 * kIh * kMw + idx] = m0;
