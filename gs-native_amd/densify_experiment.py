@@ -110,14 +110,19 @@ class Scene:
 
 
 def run(scene: Scene, iterations: int, cap: int, policy=None, outcome_log=None, eval_every: int = 0,
-        useful_opacity=0.02):
+        useful_opacity=0.02, threshold_scale: float = 1.0):
     """Trains from the scene's initial points on a context of its own (so every run pays for its
     own scratch growth and no run inherits another's allocations); returns the trainer and the
-    curve [(iteration, seconds, count, psnr)] with training wall time only (evaluations excluded)."""
+    curve [(iteration, seconds, count, psnr)] with training wall time only (evaluations excluded).
+    threshold_scale multiplies the plain rule's two gradient thresholds (the tuned-rule baseline)."""
     context = gsn.Context()
     trainer = trainer_module.Trainer(context, scene.init, scene.cameras, scene.targets, scene.extent,
                                      max_count=cap, seed=scene.seed, policy=policy, outcome_log=outcome_log,
                                      useful_opacity=useful_opacity)
+    if threshold_scale != 1.0:
+        plain = trainer.control_density
+        trainer.control_density = lambda: plain(gradient_threshold=0.0002 * threshold_scale,
+                                                absolute_threshold=0.0008 * threshold_scale)
     curve = []
     elapsed = 0.0
     context.synchronize()
@@ -308,20 +313,87 @@ def select(context, args, candidates) -> tuple[int, dict]:
                                "trials": trials, "chosen": best["candidate"]}
 
 
-def evaluate(context, args, network) -> list[dict]:
+def with_keep_margin(params: np.ndarray, margin: float) -> np.ndarray:
+    """The network with `margin` added to the keep logit's output bias (the parameter vector ends
+    with the three output biases keep, clone, split): a Gaussian is then cloned or split only where
+    that logit leads keep's by more than the margin."""
+    adjusted = np.array(params, np.float32, copy=True)
+    adjusted[-3] = np.float32(adjusted[-3] + np.float32(margin))
+    return adjusted
+
+
+def validate(context, args, params) -> tuple[float, float | None, dict]:
+    """Chooses the network's keep margin (and, with --rule-scales, the tuned-rule baseline's
+    threshold scale) on the validation scenes, none of which is used for collection or evaluation.
+    On each scene the plain rule, the rule at each threshold scale and the network at each margin
+    train once. A setting qualifies when its held-out PSNR is within the tolerance of the plain
+    rule's on every validation scene; the qualifying setting with the fewest Gaussians summed over
+    the scenes is chosen (the most accurate one, by its worst scene, if none qualifies)."""
+    margins = args.keep_margins or [args.keep_margin]
+    scales = [v for v in args.rule_scales if v != 1.0]
+    scenes = []
+    for seed in args.validation_seeds:
+        scene = scene_for(context, args, seed, args.eval_size, args.eval_truth, args.eval_init, args.eval_cameras,
+                          args.eval_heldout)
+        trainer, curve = run(scene, args.eval_iterations, args.cap, eval_every=args.eval_iterations)
+        entry = {"seed": seed, "rule_psnr": curve[-1][3], "rule_count": trainer.count, "rule_seconds": curve[-1][1],
+                 "margins": [], "scales": []}
+        log(f"validation seed {seed} rule: PSNR {curve[-1][3]:.3f} dB, {trainer.count} Gaussians, {curve[-1][1]:.1f} s")
+        del trainer
+        for margin in margins:
+            network = gsn.ProposalNetwork(context, with_keep_margin(params, margin))
+            trainer, curve = run(scene, args.eval_iterations, args.cap, policy=network, eval_every=args.eval_iterations)
+            entry["margins"].append({"margin": margin, "psnr": curve[-1][3], "count": trainer.count,
+                                     "seconds": curve[-1][1], "fallbacks": trainer.policy_fallbacks})
+            log(f"validation seed {seed} keep margin {margin}: PSNR {curve[-1][3]:.3f} dB, {trainer.count} "
+                f"Gaussians, {curve[-1][1]:.1f} s")
+            del trainer, network
+        for scale in scales:
+            trainer, curve = run(scene, args.eval_iterations, args.cap, eval_every=args.eval_iterations,
+                                 threshold_scale=scale)
+            entry["scales"].append({"scale": scale, "psnr": curve[-1][3], "count": trainer.count,
+                                    "seconds": curve[-1][1]})
+            log(f"validation seed {seed} rule threshold scale {scale}: PSNR {curve[-1][3]:.3f} dB, {trainer.count} "
+                f"Gaussians, {curve[-1][1]:.1f} s")
+            del trainer
+        scenes.append(entry)
+        del scene
+
+    def choose(key, values):
+        rows = [[next(t for t in scene[key + "s"] if t[key] == value) for scene in scenes] for value in values]
+        worst = [min(t["psnr"] - scene["rule_psnr"] for t, scene in zip(row, scenes)) for row in rows]
+        totals = [sum(t["count"] for t in row) for row in rows]
+        qualifying = [k for k in range(len(values)) if worst[k] >= -args.tolerance]
+        best = min(qualifying, key=lambda k: totals[k]) if qualifying else max(range(len(values)), key=lambda k: worst[k])
+        return values[best]
+
+    margin = choose("margin", margins)
+    scale = choose("scale", scales) if scales else None
+    return margin, scale, {"scenes": scenes, "chosen_margin": margin, "chosen_scale": scale}
+
+
+def evaluate(context, args, network, tuned_scale: float | None = None) -> list[dict]:
     evaluations = []
     for seed in range(args.first_eval_seed, args.first_eval_seed + args.eval_seeds):
         scene = scene_for(context, args, seed, args.eval_size, args.eval_truth, args.eval_init, args.eval_cameras,
                           args.eval_heldout)
         results = {}
-        for name, policy in (("rule", None), ("learned", network)):
-            trainer, curve = run(scene, args.eval_iterations, args.cap, policy=policy, eval_every=args.eval_every)
+        runs = [("rule", None, 1.0), ("learned", network, 1.0)]
+        if tuned_scale is not None:
+            runs.append(("tuned_rule", None, tuned_scale))
+        for name, policy, scale in runs:
+            trainer, curve = run(scene, args.eval_iterations, args.cap, policy=policy, eval_every=args.eval_every,
+                                 threshold_scale=scale)
             results[name] = {"curve": curve, "final_count": trainer.count, "final_psnr": curve[-1][3],
                              "seconds": curve[-1][1], "policy_fallbacks": trainer.policy_fallbacks}
             log(f"eval seed {seed} {name}: PSNR {curve[-1][3]:.3f} dB, {trainer.count} Gaussians, "
                 f"{curve[-1][1]:.1f} s")
             del trainer
-        evaluations.append(judge(seed, results["rule"], results["learned"], args.tolerance))
+        evaluation = judge(seed, results["rule"], results["learned"], args.tolerance)
+        if "tuned_rule" in results:
+            evaluation["tuned_rule"] = judge(seed, results["rule"], results["tuned_rule"], args.tolerance)
+            evaluation["tuned_rule"]["threshold_scale"] = tuned_scale
+        evaluations.append(evaluation)
         del scene
     return evaluations
 
@@ -355,7 +427,10 @@ def main() -> int:
                              "blobs: the 64-blob mixture seen from a sphere")
     parser.add_argument("--camera-radius", type=float, default=4.0)
     parser.add_argument("--train-seeds", type=int, default=10)
-    parser.add_argument("--validation-seed", type=int, default=500)
+    parser.add_argument("--validation-seed", type=int, default=500,
+                        help="the validation scene of the candidate networks' selection")
+    parser.add_argument("--validation-seeds", type=lambda text: [int(v) for v in text.split(",") if v],
+                        default=[500, 501], help="validation scenes of the keep-margin and rule-scale selection")
     parser.add_argument("--eval-seeds", type=int, default=3)
     parser.add_argument("--first-eval-seed", type=int, default=900)
     parser.add_argument("--collect-truth", type=int, default=3_000_000)
@@ -388,6 +463,14 @@ def main() -> int:
                              "initial points resolve, which the rule densifies for)")
     parser.add_argument("--params", type=str, default="",
                         help="trained network parameters (.npy): skip collection, training and selection")
+    parser.add_argument("--keep-margins", type=floats, default=[],
+                        help="keep-logit margins to choose from on the validation scene (comma-separated)")
+    parser.add_argument("--keep-margin", type=float, default=0.0,
+                        help="keep-logit margin to apply when no --keep-margins sweep is given")
+    parser.add_argument("--rule-scales", type=floats, default=[],
+                        help="gradient-threshold scales of the plain rule to choose a tuned-rule baseline from on "
+                             "the validation scene (comma-separated); the baseline then also runs on every "
+                             "evaluation scene")
     parser.add_argument("--skip-eval", action="store_true", help="collect, train and select only")
     parser.add_argument("--quick", action="store_true", help="small sizes for a functional check")
     parser.add_argument("--out", type=str, default="")
@@ -418,15 +501,19 @@ def main() -> int:
         network_params = candidates[chosen]["params"]
         if args.out:
             np.save(pathlib.Path(args.out).with_suffix(".params.npy"), network_params)
+    margin, tuned_scale = args.keep_margin, None
+    if args.keep_margins or args.rule_scales:
+        margin, tuned_scale, summary["validation"] = validate(context, args, network_params)
+    summary["keep_margin"] = margin
     if not args.skip_eval:
-        network = gsn.ProposalNetwork(context, network_params)
-        summary["evaluations"] = evaluate(context, args, network)
+        network = gsn.ProposalNetwork(context, with_keep_margin(network_params, margin))
+        summary["evaluations"] = evaluate(context, args, network, tuned_scale)
         summary["accepted_all"] = all(e["accepted"] for e in summary["evaluations"])
     if args.out:
         pathlib.Path(args.out).write_text(json.dumps(summary, indent=1))
     if args.record and "evaluations" in summary:
         first = summary["evaluations"][0]
-        record = {"config": summary["config"], "held_out": {
+        record = {"config": summary["config"], "keep_margin": margin, "held_out": {
             "seed": first["seed"], "accepted": first["accepted"], "psnr_delta_db": first["psnr_delta_db"],
             "rule_psnr": first["rule"]["final_psnr"], "learned_psnr": first["learned"]["final_psnr"],
             "rule_count": first["rule"]["final_count"], "learned_count": first["learned"]["final_count"]}}

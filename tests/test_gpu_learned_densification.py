@@ -5,8 +5,9 @@ Gaussians, 1920x1080, 300 training and 8 held-out cameras on a radius-4 hemisphe
 200,000 points sampled from the scene (reference initialisation) for 7,000 iterations of the
 reference schedule with a 6,000,000-Gaussian cap. The proposal network is the committed one
 (gs-native_amd/data/densify_proposal.npy: collected, trained and chosen on a validation scene by
-densify_experiment.py; its record, densify_proposal.json, holds what that run measured on this
-seed). Asserts, on held-out seed 900:
+densify_experiment.py; its record, densify_proposal.json, holds the keep-logit margin chosen on two
+validation scenes and what the evaluation measured on the first held-out seed, 903). Asserts, on
+that seed:
 - the judge's decision (learned held-out PSNR within 0.05 dB of the rule's, or not) is the
   recorded one;
 - the PSNR difference learned - rule is within 0.05 dB of the recorded difference (the gain, or
@@ -43,7 +44,7 @@ def test_learned_policy_against_the_rule_on_a_held_out_scene(gsn, context):
     config = record["config"]
     experiment = _experiment(gsn)
     params = np.load(DATA / "densify_proposal.npy", allow_pickle=False).astype(np.float32)
-    network = gsn.ProposalNetwork(context, params)
+    network = gsn.ProposalNetwork(context, experiment.with_keep_margin(params, record.get("keep_margin", 0.0)))
     width, height = (int(v) for v in config["eval_size"].split("x"))
     scene = experiment.Scene(context, expected["seed"], config["eval_truth"], config["eval_init"], width, height,
                              config["eval_cameras"], config["eval_heldout"], config["colour_noise"],
