@@ -248,6 +248,8 @@ def main() -> int:
                         help="iterations of the full-schedule run from sparse points (0 skips it)")
     parser.add_argument("--full-schedule-points", type=int, default=200_000)
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-backward-overlap", action="store_true",
+                        help="prepare the backward on the main stream (no second stream)")
     parser.add_argument("--cpu-runs", type=int, default=5)
     parser.add_argument("--calibration-copy-mib", type=int, default=0,
                         help="after the timed steps, run one device-to-device copy of this many MiB (a plain copy "
@@ -283,6 +285,7 @@ def main() -> int:
     t = ((args.width + 15) // 16) * ((args.height + 15) // 16)
 
     trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=start)
+    trainer.overlap_backward = not args.no_backward_overlap
     for _ in range(args.warmup):
         trainer.iteration()
     context.synchronize()
@@ -439,6 +442,7 @@ def main() -> int:
     # 3. The same state entered just before the iteration-3000 opacity cap.
     if not args.no_post_reset:
         reset_trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=RESET_ITERATION - 1)
+        reset_trainer.overlap_backward = not args.no_backward_overlap
         reset_trainer.iteration()  # iteration 3000: the 0.01 opacity cap is applied after it
         for _ in range(args.warmup):
             reset_trainer.iteration()
@@ -479,6 +483,7 @@ def main() -> int:
         points = scenes.blob_scene(args.full_schedule_points, seed=0, sh_degree=0)
         init = workloads.initial_from_points(gsn, context, points)
         full = trainer_module.Trainer(context, init, cameras, targets, extent, max_count=6_000_000)
+        full.overlap_backward = not args.no_backward_overlap
         context.synchronize()
         context.set_timing(True, stages=["density"])
         begin = time.perf_counter()

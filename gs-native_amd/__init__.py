@@ -147,6 +147,8 @@ _PROTOTYPES = {
     "gsn_context_stage_times": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I), _I],
     "gsn_context_count_raster_pairs": [_P, _I],
     "gsn_context_raster_pairs": [_P, ctypes.POINTER(ctypes.c_ulonglong)],
+    "gsn_context_prefetch_backward": [_P],
+    "gsn_context_prefetch_reused": [_P, ctypes.POINTER(ctypes.c_ulonglong)],
 }
 
 STAGES = ("project", "scan", "depth_sort", "emit", "tile_sort", "ranges", "raster_forward", "image_gradients",
@@ -345,6 +347,17 @@ class Context:
         """Evaluated (pixel, splat) pairs since the last read (clears the counter)."""
         value = ctypes.c_ulonglong(0)
         call("gsn_context_raster_pairs", self._handle, ctypes.byref(value))
+        return int(value.value)
+
+    def prefetch_backward(self) -> None:
+        """Arms the next tile render of the last projection to queue the backward's preparation on
+        a second stream (overlapped with the forward raster and the image gradient)."""
+        call("gsn_context_prefetch_backward", self._handle)
+
+    def prefetch_reused(self) -> int:
+        """Gradient calls on this context that used work queued by an armed render."""
+        value = ctypes.c_ulonglong(0)
+        call("gsn_context_prefetch_reused", self._handle, ctypes.byref(value))
         return int(value.value)
 
     def close(self) -> None:

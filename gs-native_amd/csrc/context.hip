@@ -50,6 +50,9 @@ void* scratch(gsn_context* context, ScratchSlot slot, size_t bytes) {
     grown = (grown + 255) & ~static_cast<size_t>(255);
     if (context->slots[slot]) {
         (void)hipStreamSynchronize(context->stream);
+        if (context->aux_stream) {
+            (void)hipStreamSynchronize(context->aux_stream);
+        }
         (void)hipFree(context->slots[slot]);
         context->slots[slot] = nullptr;
         context->slot_bytes[slot] = 0;
@@ -112,6 +115,16 @@ void gsn_context_destroy(gsn_context* context) {
     if (context->stream) {
         (void)hipStreamSynchronize(context->stream);
     }
+    if (context->aux_stream) {
+        (void)hipStreamSynchronize(context->aux_stream);
+        (void)hipStreamDestroy(context->aux_stream);
+    }
+    if (context->fork_event) {
+        (void)hipEventDestroy(context->fork_event);
+    }
+    if (context->join_event) {
+        (void)hipEventDestroy(context->join_event);
+    }
     for (int slot = 0; slot < kSlotCount; ++slot) {
         if (context->slots[slot]) {
             (void)hipFree(context->slots[slot]);
@@ -146,6 +159,9 @@ int gsn_device_synchronize(void) {
 int gsn_context_synchronize(gsn_context* context) {
     if (!context) {
         return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_context_synchronize: null context");
+    }
+    if (context->aux_stream) {
+        GSN_HIP(hipStreamSynchronize(context->aux_stream));
     }
     return gsn::check_hip(hipStreamSynchronize(context->stream), "hipStreamSynchronize");
 }
@@ -405,6 +421,28 @@ extern "C" int gsn_context_count_raster_pairs(gsn_context* context, int enabled)
         GSN_HIP(hipMemsetAsync(context->raster_pairs, 0, sizeof(unsigned long long), context->stream));
     }
     context->count_raster_pairs = enabled != 0;
+    return GSN_OK;
+}
+
+extern "C" int gsn_context_prefetch_backward(gsn_context* context) {
+    if (!context) {
+        return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_context_prefetch_backward: null context");
+    }
+    if (!context->aux_stream) {
+        // Default priority, like the main stream.
+        GSN_HIP(hipStreamCreate(&context->aux_stream));
+        GSN_HIP(hipEventCreateWithFlags(&context->fork_event, hipEventDisableTiming));
+        GSN_HIP(hipEventCreateWithFlags(&context->join_event, hipEventDisableTiming));
+    }
+    context->prefetch_armed = true;
+    return GSN_OK;
+}
+
+extern "C" int gsn_context_prefetch_reused(gsn_context* context, unsigned long long* calls) {
+    if (!context || !calls) {
+        return gsn::fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_context_prefetch_reused: invalid arguments");
+    }
+    *calls = context->prefetch_reused;
     return GSN_OK;
 }
 

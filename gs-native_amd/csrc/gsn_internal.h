@@ -63,6 +63,21 @@ enum Stage {
     kStageCount
 };
 
+// The backward's splat copy and schedules queued on the auxiliary stream by an armed
+// gsn_render_tiles, and the buffers they were computed from.
+struct BackwardPrefetch {
+    const float* means = nullptr;
+    const float* conics = nullptr;
+    const float* colors = nullptr;
+    const int* rects = nullptr;
+    const int* offsets = nullptr;
+    const int64_t* tile_offsets = nullptr;
+    int count = -1;
+    int tiles_x = 0;
+    int tiles_y = 0;
+    unsigned generation = 0;
+};
+
 struct gsn_context {
     hipStream_t stream = nullptr;
     int device = 0;
@@ -82,6 +97,26 @@ struct gsn_context {
     const float* projected_depths = nullptr;
     const int* projected_offsets = nullptr;
     int projected_count = -1;
+    // The last projection's other outputs, and a count of projections.
+    const float* projected_means = nullptr;
+    const float* projected_conics = nullptr;
+    const float* projected_colors = nullptr;
+    const int* projected_rects = nullptr;
+    const int* projected_record_offsets = nullptr;  // kept when the depth-order fields are consumed
+    int projected_gaussians = -1;
+    unsigned projection_generation = 0;
+    // Backward prefetch. gsn_context_prefetch_backward arms the next gsn_render_tiles of the last
+    // projection: it queues the backward's splat packing and schedules on aux_stream behind
+    // fork_event (the main stream's work so far) and records join_event. The gradient entry
+    // points, the next projection and the next prefetch make the main stream wait for join_event
+    // first; the gradient entry points reuse the queued work when their buffers are the prefetch's.
+    bool prefetch_armed = false;
+    bool prefetch_pending = false;
+    unsigned long long prefetch_reused = 0;  // gradient calls that used queued work (diagnostic)
+    BackwardPrefetch prefetch;
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t fork_event = nullptr;
+    hipEvent_t join_event = nullptr;
     const float* ordered_depths = nullptr;
     const int* ordered_offsets = nullptr;
     int ordered_count = -1;
@@ -174,6 +209,12 @@ struct StoredState {
 // before all its pixels are done get a partial; for each of them the backward also stores the
 // step's epoch into record_flags[slot] (*flags, *epoch are returned), so the per-Gaussian pass
 // reads exactly the partials written in this step.
+// Queues the backward's splat copy and schedules of the last projection on the auxiliary stream
+// (an armed gsn_render_tiles; raster_backward.hip).
+int queue_backward_prefetch(gsn_context* context, const Camera& cam, const int64_t* tile_offsets);
+// Makes the main stream wait for queued prefetch work (nothing if none is pending).
+int join_backward_prefetch(gsn_context* context);
+
 int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
     const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
     const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,

@@ -204,9 +204,18 @@ extern "C" int gsn_project_gaussians(gsn_context* context, const float* params, 
     context->ordered_depths = nullptr;
     context->ordered_offsets = nullptr;
     context->ordered_count = -1;
+    // Queued backward preparation of the previous projection may still read its outputs.
+    GSN_TRY(join_backward_prefetch(context));
     context->projected_depths = count > 0 ? depths : nullptr;
     context->projected_offsets = record_offsets;
     context->projected_count = count;
+    context->projected_means = means;
+    context->projected_conics = conic_opacities;
+    context->projected_colors = colors;
+    context->projected_rects = tile_rects;
+    context->projected_record_offsets = record_offsets;
+    context->projected_gaussians = count;
+    ++context->projection_generation;
     if (count == 0) {
         return check_hip(hipMemsetAsync(record_offsets, 0, sizeof(int), context->stream), "hipMemsetAsync");
     }

@@ -524,6 +524,75 @@ __global__ __launch_bounds__(256) void clear_flags_kernel(uint32_t* flags, int64
 
 } // namespace
 
+// The backward's splat copy and its two schedules (tile order, per-Gaussian group order).
+struct BackwardBuffers {
+    PackedSplat* splats = nullptr;
+    int* order = nullptr;
+    int* groups_order = nullptr;
+};
+
+static int backward_buffers(gsn_context* context, const Camera& cam, int count, BackwardBuffers* buffers) {
+    const int64_t tiles = static_cast<int64_t>(cam.tile_count_x) * cam.tile_count_y;
+    const int groups = (count + kGradientGroup - 1) / kGradientGroup;
+    buffers->splats = static_cast<PackedSplat*>(
+        scratch(context, kSlotSplats, static_cast<size_t>(count > 0 ? count : 1) * sizeof(PackedSplat)));
+    buffers->order = static_cast<int*>(scratch(context, kSlotTileOrder, static_cast<size_t>(tiles) * 4));
+    buffers->groups_order =
+        static_cast<int*>(scratch(context, kSlotGaussianOrder, static_cast<size_t>(groups > 0 ? groups : 1) * 4));
+    return buffers->splats && buffers->order && buffers->groups_order ? GSN_OK : GSN_ERROR_OUT_OF_MEMORY;
+}
+
+static int launch_backward_preparation(hipStream_t stream, const Camera& cam, int count, const int64_t* tile_offsets,
+    const float* means, const float* conics, const float* colors, const int* rects, const int* record_offsets,
+    const BackwardBuffers& buffers) {
+    const int64_t tiles = static_cast<int64_t>(cam.tile_count_x) * cam.tile_count_y;
+    pack_splats_kernel<<<grid_for(count, 256), 256, 0, stream>>>(reinterpret_cast<const float2*>(means),
+        reinterpret_cast<const float4*>(conics), colors, reinterpret_cast<const int4*>(rects), record_offsets, count,
+        buffers.splats);
+    GSN_TRY(check_launch("pack_splats_kernel"));
+    tile_schedule_kernel<<<2, kScheduleThreads, 0, stream>>>(tile_offsets, static_cast<int>(tiles), buffers.order,
+        record_offsets, count, buffers.groups_order);
+    return check_launch("tile_schedule_kernel");
+}
+
+int join_backward_prefetch(gsn_context* context) {
+    if (!context->prefetch_pending) {
+        return GSN_OK;
+    }
+    context->prefetch_pending = false;
+    return check_hip(hipStreamWaitEvent(context->stream, context->join_event, 0), "hipStreamWaitEvent");
+}
+
+int queue_backward_prefetch(gsn_context* context, const Camera& cam, const int64_t* tile_offsets) {
+    GSN_TRY(join_backward_prefetch(context));
+    const int count = context->projected_gaussians;
+    if (count <= 0 || !context->aux_stream || cam.tile_count_x > 0xffff || cam.tile_count_y > 0x7fff) {
+        return GSN_OK;
+    }
+    // Allocations first: growing a slot synchronises both streams before it frees the old buffer.
+    BackwardBuffers buffers;
+    GSN_TRY(backward_buffers(context, cam, count, &buffers));
+    GSN_HIP(hipEventRecord(context->fork_event, context->stream));
+    GSN_HIP(hipStreamWaitEvent(context->aux_stream, context->fork_event, 0));
+    GSN_TRY(launch_backward_preparation(context->aux_stream, cam, count, tile_offsets, context->projected_means,
+        context->projected_conics, context->projected_colors, context->projected_rects,
+        context->projected_record_offsets, buffers));
+    GSN_HIP(hipEventRecord(context->join_event, context->aux_stream));
+    context->prefetch_pending = true;
+    BackwardPrefetch& queued = context->prefetch;
+    queued.means = context->projected_means;
+    queued.conics = context->projected_conics;
+    queued.colors = context->projected_colors;
+    queued.rects = context->projected_rects;
+    queued.offsets = context->projected_record_offsets;
+    queued.tile_offsets = tile_offsets;
+    queued.count = count;
+    queued.tiles_x = cam.tile_count_x;
+    queued.tiles_y = cam.tile_count_y;
+    queued.generation = context->projection_generation;
+    return GSN_OK;
+}
+
 int launch_splat_backward(gsn_context* context, const Camera& cam, int count, const int* ids,
     const int64_t* tile_offsets, const float* means, const float* conics, const float* colors, const int* rects,
     const int* record_offsets, const float* image, const float* image_gradients, float* partials, int record_count,
@@ -532,11 +601,20 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
         return fail(GSN_ERROR_INVALID_ARGUMENT, "raster backward: tile grid exceeds 65535 x 32767");
     }
     int64_t tiles = static_cast<int64_t>(cam.tile_count_x) * cam.tile_count_y;
-    PackedSplat* splats = static_cast<PackedSplat*>(
-        scratch(context, kSlotSplats, static_cast<size_t>(count > 0 ? count : 1) * sizeof(PackedSplat)));
-    int* order = static_cast<int*>(scratch(context, kSlotTileOrder, static_cast<size_t>(tiles) * 4));
-    const int groups = (count + kGradientGroup - 1) / kGradientGroup;
-    int* groups_order = static_cast<int*>(scratch(context, kSlotGaussianOrder, static_cast<size_t>(groups > 0 ? groups : 1) * 4));
+    // Work queued by an armed gsn_render_tiles is joined first, and reused when it was computed
+    // from these buffers of the current projection.
+    const BackwardPrefetch& queued = context->prefetch;
+    const bool prefetched = context->prefetch_pending && queued.generation == context->projection_generation &&
+        queued.means == means && queued.conics == conics && queued.colors == colors && queued.rects == rects &&
+        queued.offsets == record_offsets && queued.tile_offsets == tile_offsets && queued.count == count &&
+        queued.tiles_x == cam.tile_count_x && queued.tiles_y == cam.tile_count_y;
+    GSN_TRY(join_backward_prefetch(context));
+    context->prefetch = BackwardPrefetch{};
+    BackwardBuffers buffers;
+    GSN_TRY(backward_buffers(context, cam, count, &buffers));
+    PackedSplat* splats = buffers.splats;
+    int* order = buffers.order;
+    int* groups_order = buffers.groups_order;
     // One flag byte per record, read 16 at a time by the per-Gaussian pass (scratch slots are
     // allocated in multiples of 256 bytes, so an aligned 16-byte read below the record count
     // stays inside the allocation).
@@ -562,14 +640,14 @@ int launch_splat_backward(gsn_context* context, const Camera& cam, int count, co
     *flags = record_flags;
     *epoch = next;
     StageScope scope(context, kStageRasterBackward);
-    pack_splats_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(reinterpret_cast<const float2*>(means),
-        reinterpret_cast<const float4*>(conics), colors, reinterpret_cast<const int4*>(rects), record_offsets, count,
-        splats);
-    GSN_TRY(check_launch("pack_splats_kernel"));
-    tile_schedule_kernel<<<2, kScheduleThreads, 0, context->stream>>>(tile_offsets, static_cast<int>(tiles), order,
-        record_offsets, count, groups_order);
+    if (prefetched) {
+        ++context->prefetch_reused;
+    }
+    else {
+        GSN_TRY(launch_backward_preparation(context->stream, cam, count, tile_offsets, means, conics, colors, rects,
+            record_offsets, buffers));
+    }
     *gaussian_order = groups_order;
-    GSN_TRY(check_launch("tile_schedule_kernel"));
     if (stored) {
         raster_backward_stored_kernel<<<static_cast<unsigned>(tiles), 64, 0, context->stream>>>(ids, tile_offsets,
             splats, order, cam, stored->final_transmittance, stored->last_contributor, image_gradients,

@@ -157,6 +157,15 @@ int render_tiles(gsn_context* context, const gsn_camera* camera, const int* tile
     Camera cam;
     memcpy(&cam, camera, sizeof(cam));
     unsigned long long* pair_count = context->count_raster_pairs ? context->raster_pairs : nullptr;
+    if (context->prefetch_armed) {
+        // Armed by gsn_context_prefetch_backward: the backward's preparation of this projection
+        // runs on the auxiliary stream beside this kernel and the image gradient.
+        context->prefetch_armed = false;
+        if (means == context->projected_means && conic_opacities == context->projected_conics &&
+            colors == context->projected_colors) {
+            GSN_TRY(queue_backward_prefetch(context, cam, tile_offsets));
+        }
+    }
     StageScope scope(context, kStageRasterForward);
     const unsigned grid = static_cast<unsigned>(tiles);
     const float2* m = reinterpret_cast<const float2*>(means);

@@ -115,6 +115,7 @@ class Trainer:
         self.adam_step = 0
         self.sh_degree = 0
         self.projection = None
+        self.overlap_backward = True
         self.rng = np.random.default_rng(seed)
         self.order = np.arange(len(cameras))
         self.cursor = len(cameras)
@@ -163,6 +164,10 @@ class Trainer:
         iteration = self.completed + 1
         self.sh_degree = min(3, iteration // 1000)
         lr = self.position_learning_rate(iteration)
+        if self.overlap_backward:
+            # The backward's preparation then runs on a second stream beside the forward raster
+            # and the image gradient; results are identical either way.
+            self.context.prefetch_backward()
         self.projection = render(self.context, self.params, self.count, camera, self.background, self.image,
                                  self.sh_degree, self.projection)
         image_gradients(self.context, camera, self.image, self.targets[index], self.image_grad)

@@ -104,6 +104,20 @@ int gsn_context_stage_times(gsn_context* context, double* milliseconds, int* lau
 int gsn_context_count_raster_pairs(gsn_context* context, int enabled);
 int gsn_context_raster_pairs(gsn_context* context, unsigned long long* pairs);
 
+/* Overlap for training loops: arms the next gsn_render_tiles / gsn_render_tiles_stored of the
+ * context's last projection to also queue the splat backward's preparation (a packed copy of the
+ * projected splats and the two work schedules, which read only the projection and tile lists) on a
+ * second, default-priority stream of the context. It then runs beside the forward raster and the
+ * image gradient, and the next gradient entry point of that projection uses it instead of
+ * preparing again. Every gradient entry point, the next projection and gsn_context_synchronize
+ * wait for the queued work first. Until one of them has run, the projection's output buffers must
+ * stay allocated. One-shot: a render it does not arm does no extra work. Results are identical
+ * with and without it. No counterpart in the reference (one stream throughout). */
+int gsn_context_prefetch_backward(gsn_context* context);
+/* Diagnostic: the number of gradient calls on this context that used work queued by an armed
+ * render. No counterpart in the reference. */
+int gsn_context_prefetch_reused(gsn_context* context, unsigned long long* calls);
+
 /* Diagnostic: the wave64 reduction used by the raster backward, on one wave.
  * input float[64][12] -> output float[64][3]; lane l receives the sums over all lanes of
  * fields 3*(l/16) .. 3*(l/16)+2. No counterpart in the reference. */
