@@ -224,6 +224,12 @@ bool GaussianSplatting::Step() {
     const int iteration = completed_ + 1;
     const schedule::Step plan = schedule::StepFor(iteration, scene_extent_);
     sh_degree_ = plan.sh_degree;
+    // The solver's splat backward then prepares on a second stream beside the forward raster.
+    gsn_context* context = DefaultContext();
+    if (context && gsn_context_prefetch_backward(context) != GSN_OK) {
+        GS_NATIVE_ERROR("arming the backward prefetch failed: %s", gsn_last_error());
+        return false;
+    }
     if (!renderer_.Render(gaussians_, camera, background_, image_, sh_degree_) ||
         !solver_.Iterate(camera, renderer_, image_, targets_[static_cast<size_t>(view)], gaussians_,
             plan.position_learning_rate, sh_degree_)) {
