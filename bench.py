@@ -99,6 +99,9 @@ def algorithmic_bytes(*, n: float, v: float, r: float, p: int, t: int, k: int = 
         "image_gradients": 27 * p,
         "raster_backward": 40 * r + 24 * p + 44 * v,
         "gaussian_gradients": 1412 * n + r,
+        # with the next view's projection fused in: its outputs (record count for every Gaussian;
+        # mean, conic + opacity, colour, depth, rectangle for the visible ones); it reads nothing more
+        "gaussian_gradients_projected": 1416 * n + 56 * v + r,
         "gaussian_gradients_unfused": 484 * n + r,  # params read, gradients written, statistics, offsets, flags
         "adam": 1624 * n,                           # params, moments read + written, gradients read
         "accumulate": 24 * n,
@@ -250,6 +253,8 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-backward-overlap", action="store_true",
                         help="prepare the backward on the main stream (no second stream)")
+    parser.add_argument("--no-fused-projection", action="store_true",
+                        help="project in each render instead of in the previous gradient pass")
     parser.add_argument("--cpu-runs", type=int, default=5)
     parser.add_argument("--calibration-copy-mib", type=int, default=0,
                         help="after the timed steps, run one device-to-device copy of this many MiB (a plain copy "
@@ -286,6 +291,7 @@ def main() -> int:
 
     trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=start)
     trainer.overlap_backward = not args.no_backward_overlap
+    trainer.fuse_projection = not args.no_fused_projection
     for _ in range(args.warmup):
         trainer.iteration()
     context.synchronize()
@@ -373,7 +379,12 @@ def main() -> int:
     window_bytes = algorithmic_bytes(n=n_mean, v=v_mean, r=r_mean, p=p, t=t)
     launches = timed[dominant][1]
     avg_ms = timed[dominant][0] / launches
-    achieved = window_bytes[dominant] / (avg_ms * 1e-3) / 1e9
+    projected = trainer.fuse_projection
+    dominant_bytes = window_bytes["gaussian_gradients_projected" if projected else dominant]
+    # The same work as the reference's separate kernels: statistics, chain rule and Adam (and, with
+    # the projection fused in, the projection kernel's reads and writes).
+    separate_bytes = unfused_equivalent_bytes(n_mean, v_mean) + (window_bytes["project"] if projected else 0.0)
+    achieved = dominant_bytes / (avg_ms * 1e-3) / 1e9
     density_ms, density_launches = timed["density"]
     density_steps = trainer.history[history_before:]
     traffic, traffic_source = profiled_value("traffic_bytes_per_launch", dominant)
@@ -411,7 +422,8 @@ def main() -> int:
         "stage_pass": {"iterations": stage_steps, "note": per_stage_note},
         "raster": raster,
         "roofline": {
-            "kernel": "gaussian_gradients_kernel (per-Gaussian gradients + chain rule + Adam, fused)",
+            "kernel": "gaussian_gradients_kernel (per-Gaussian gradients + chain rule + Adam"
+                      + (" + the next view's projection, fused)" if projected else ", fused)"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -421,11 +433,12 @@ def main() -> int:
             "traffic_source": traffic_source,
             "avg_launch_ms": avg_ms,
             "launches": launches,
-            "algorithmic_bytes_per_launch": window_bytes[dominant],
-            "bytes_note": "fused floor 1412N + R (Adam rows read + written, statistics, run offsets, record flags); "
-                          "the partials of reached records are extra",
-            "unfused_equivalent_bytes_per_launch": unfused_equivalent_bytes(n_mean, v_mean),
-            "unfused_equivalent_frac": unfused_equivalent_bytes(n_mean, v_mean) / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_launch": dominant_bytes,
+            "bytes_note": "fused floor 1412N + R (Adam rows read + written, statistics, run offsets, record flags)"
+                          + (" + 4N + 56V (the next view's projection outputs; V = mean visible Gaussians)"
+                             if projected else "") + "; the partials of reached records are extra",
+            "unfused_equivalent_bytes_per_launch": separate_bytes,
+            "unfused_equivalent_frac": separate_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": None,
     }
@@ -443,6 +456,7 @@ def main() -> int:
     if not args.no_post_reset:
         reset_trainer = trainer_module.Trainer(context, params, cameras, targets, extent, completed=RESET_ITERATION - 1)
         reset_trainer.overlap_backward = not args.no_backward_overlap
+        reset_trainer.fuse_projection = not args.no_fused_projection
         reset_trainer.iteration()  # iteration 3000: the 0.01 opacity cap is applied after it
         for _ in range(args.warmup):
             reset_trainer.iteration()
@@ -484,6 +498,7 @@ def main() -> int:
         init = workloads.initial_from_points(gsn, context, points)
         full = trainer_module.Trainer(context, init, cameras, targets, extent, max_count=6_000_000)
         full.overlap_backward = not args.no_backward_overlap
+        full.fuse_projection = not args.no_fused_projection
         context.synchronize()
         context.set_timing(True, stages=["density"])
         begin = time.perf_counter()

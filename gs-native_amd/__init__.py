@@ -110,6 +110,8 @@ _PROTOTYPES = {
     "gsn_gaussian_gradients": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
                                     _F, _F, _F, _P, _P],
+    "gsn_gaussian_gradients_adam_project": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
+                                            _F, _F, _F, _P, _P, _CAM, _P, _P, _P, _P, _P, _P],
     "gsn_adam_update": [_P, _I, _P, _P, _P, _F, _F, _F],
     "gsn_remap_history": [_P, _P, _I, _I, _P, _P],
     "gsn_zero_rows": [_P, _P, _I, _I, _I],
@@ -445,6 +447,19 @@ def render(context: Context, params: DeviceBuffer, count: int, camera: Camera, b
     call("gsn_project_gaussians", h, _p(params), count, ctypes.byref(camera), sh_degree, _p(projection.means),
          _p(projection.conic_opacities), _p(projection.colors), _p(projection.depths), _p(projection.tile_rects),
          _p(projection.record_offsets))
+    return render_projected(context, projection, background, image, keep_keys, stored_state)
+
+
+def render_projected(context: Context, projection: Projection, background, image: DeviceBuffer,
+                     keep_keys: bool = False, stored_state: tuple | None = None) -> Projection:
+    """The rest of render() for a projection whose per-Gaussian outputs are written (by
+    gsn_project_gaussians or by the fused gradient pass): record scan -> tile lists -> raster
+    forward."""
+    h = context.handle
+    camera = projection.camera
+    count = projection.count
+    if projection.tile_offsets.shape[0] < camera.tile_count + 1:
+        projection.tile_offsets = DeviceBuffer((camera.tile_count + 1,), np.int64)
     records = ctypes.c_int(0)
     call("gsn_scan_record_offsets", h, _p(projection.record_offsets), count, ctypes.byref(records))
     projection.record_count = records.value
@@ -560,6 +575,30 @@ def gaussian_gradients_adam(context: Context, projection: Projection, params: De
          _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets), _p(p.tile_rects),
          _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(moments),
          position_lr, first_correction, second_correction, _p(statistics), _p(absolute_statistics))
+
+
+def gaussian_gradients_adam_project(context: Context, projection: Projection, params: DeviceBuffer, sh_degree: int,
+                                    image: DeviceBuffer, image_grad: DeviceBuffer, moments: Moments,
+                                    position_lr: float, first_correction: float, second_correction: float,
+                                    statistics: DeviceBuffer, absolute_statistics: DeviceBuffer, next_camera: Camera,
+                                    next_projection: Projection | None = None) -> Projection:
+    """gaussian_gradients_adam, then the projection of the updated parameters for next_camera,
+    fused (see gsn_gaussian_gradients_adam_project). Returns next_projection (allocated when None)
+    with its per-Gaussian outputs written; finish the render with render_projected."""
+    p = projection
+    if next_projection is None or next_projection is projection:
+        next_projection = Projection(p.count, next_camera)
+    elif next_projection.count != p.count:
+        next_projection.resize(p.count)
+    next_projection.camera = next_camera
+    q = next_projection
+    call("gsn_gaussian_gradients_adam_project", context.handle, ctypes.byref(p.camera), p.count, sh_degree,
+         _p(params), _p(p.tile_gaussian_ids), _p(p.tile_offsets), p.record_count, _p(p.record_offsets),
+         _p(p.tile_rects), _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(moments),
+         position_lr, first_correction, second_correction, _p(statistics), _p(absolute_statistics),
+         ctypes.byref(next_camera), _p(q.means), _p(q.conic_opacities), _p(q.colors), _p(q.depths), _p(q.tile_rects),
+         _p(q.record_offsets))
+    return next_projection
 
 
 def adam_corrections(step: int) -> tuple[float, float]:

@@ -6,6 +6,7 @@
 //   CollectProjectionStatisticsKernel 468-490, MapGaussianGradientsKernel<D> 492-623
 //   (ComputeEquationRotation 185-221, ComputeEquationSh<D> 223-323),
 //   UpdateGaussiansKernel 651-689, RemapGaussianHistoryKernel 625-649.
+#include "projection.h"
 #include "raster_common.h"
 
 #include <cmath>
@@ -458,19 +459,38 @@ __device__ __forceinline__ void reduce_records(const float* __restrict__ partial
     }
 }
 
-template<int Degree, bool Fused, bool Streaming>
+// The next view's projection outputs, written by the fused pass from the updated parameters.
+struct NextProjection {
+    Camera camera;
+    float2* means;
+    float4* conic_opacities;
+    float* colors;
+    float* depths;
+    int4* tile_rects;
+    int* record_counts;
+};
+
+// Project: with Fused, also project each Gaussian's updated parameters for the next view
+// (project_geometry after the first Adam group, which holds rows 0-15; each SH coefficient's
+// colour term as its row is updated, in coefficient order), so the next iteration reads no
+// parameter rows to project. The outputs equal the projection kernel's bit for bit.
+template<int Degree, bool Fused, bool Streaming, bool Project>
 __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(const int* __restrict__ group_order,
     const float* __restrict__ partials,
     const int* __restrict__ record_offsets, const uint8_t* __restrict__ record_flags, uint8_t epoch, int count,
     int record_count, Camera cam, float* params, float* gradients, float* screen, float* statistics,
     float* absolute_statistics, float* moments, float position_learning_rate, float first_correction,
-    float second_correction) {
+    float second_correction, NextProjection next) {
+    static_assert(Fused || !Project, "the next view is projected from the Adam update");
     __shared__ float4 s_records[kGradientBlock / 64][kChunks * 64 * 3];
     const int wave = threadIdx.x >> 6;
     // Workgroups run heaviest record run first (group_order); results do not depend on the order.
     const int group = group_order ? group_order[blockIdx.x] : static_cast<int>(blockIdx.x);
     const int id = group * kGradientBlock + threadIdx.x;
     const int64_t n = count;
+    if (Project && id == 0) {
+        next.record_counts[count] = 0;
+    }
     float in[11];
     int begin, end;
     reduce_records(partials, record_offsets, record_flags, epoch, id, count, record_count, s_records[wave], in, begin,
@@ -521,6 +541,10 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
         // lines, which ordinary accesses share with the neighbouring wave through the cache.
         float* __restrict__ p = params;
         float* __restrict__ mv = moments;
+        constexpr int coeffs = (Degree + 1) * (Degree + 1);
+        float basis[kShCoeffCount];
+        float colour[3] = {0.0f, 0.0f, 0.0f};
+        bool visible = false;
 #pragma unroll
         for (int base = 0; base < kParamCount; base += kAdamGroup) {
             float pv[kAdamGroup];
@@ -549,6 +573,28 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
                     }
                     store_moments(mv, moment_index(id, row), m, v);
                 }
+            }
+            if (Project) {
+                static_assert(kAdamGroup >= kShIndex, "the first group holds the geometry rows");
+                if (base == 0) {
+                    next.record_counts[id] = 0;
+                    visible = project_geometry<Degree>([&](int r) { return pv[r]; }, next.camera, id, next.means,
+                        next.conic_opacities, next.depths, next.tile_rects, next.record_counts, basis);
+                }
+#pragma unroll
+                for (int k = 0; k < kAdamGroup; ++k) {
+                    const int row = base + k;
+                    if (row >= kShIndex && row < kParamCount && (row - kShIndex) % kShCoeffCount < coeffs) {
+                        const int channel = (row - kShIndex) / kShCoeffCount;
+                        colour[channel] = add_sh_term(colour[channel], basis[(row - kShIndex) % kShCoeffCount], pv[k]);
+                    }
+                }
+            }
+        }
+        if (Project && visible) {
+#pragma unroll
+            for (int channel = 0; channel < 3; ++channel) {
+                next.colors[static_cast<int64_t>(id) * 3 + channel] = finish_colour(colour[channel]);
             }
         }
     }
@@ -646,67 +692,63 @@ __global__ __launch_bounds__(256) void zero_rows_kernel(float* moments, int coun
     }
 }
 
+template<int Degree, bool Fused, bool Project>
+void launch_gradients_degree(bool streaming, unsigned grid, hipStream_t stream, const int* group_order,
+    const float* partials, const int* record_offsets, const uint8_t* record_flags, uint8_t epoch, int count,
+    int record_count, const Camera& cam, float* params, float* gradients, float* screen, float* statistics,
+    float* absolute_statistics, float* moments, float lr, float c1, float c2, const NextProjection& next) {
+    if (Fused && streaming) {
+        gaussian_gradients_kernel<Degree, Fused, Fused, Project><<<grid, kGradientBlock, 0, stream>>>(group_order,
+            partials, record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen,
+            statistics, absolute_statistics, moments, lr, c1, c2, next);
+    }
+    else {
+        gaussian_gradients_kernel<Degree, Fused, false, Project><<<grid, kGradientBlock, 0, stream>>>(group_order,
+            partials, record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen,
+            statistics, absolute_statistics, moments, lr, c1, c2, next);
+    }
+}
+
+// next: with Fused, the next view's projection to write from the updated parameters (or null).
 template<bool Fused>
 int launch_gradients(gsn_context* context, const Camera& cam, int count, int record_count, int sh_degree,
     const int* group_order, const float* partials, const int* record_offsets, const uint8_t* record_flags,
     uint8_t epoch, float* params, float* gradients, float* screen, float* statistics, float* absolute_statistics,
-    float* moments, float lr, float c1, float c2) {
+    float* moments, float lr, float c1, float c2, const NextProjection* next = nullptr) {
     unsigned grid = grid_for(count, kGradientBlock);
     hipStream_t stream = context->stream;
     // Non-temporal streaming of the parameter rows only when all of them start on a 128-byte line
     // (the blocked moments always stream non-temporally).
     const bool streaming = Fused && rows_line_aligned(params, count);
+    const NextProjection none{};
+    const NextProjection& target = next ? *next : none;
     StageScope scope(context, kStageGaussianGradients);
+#define GSN_LAUNCH_DEGREE(D)                                                                                           \
+    if (Fused && next) {                                                                                               \
+        launch_gradients_degree<D, Fused, Fused>(streaming, grid, stream, group_order, partials, record_offsets,        \
+            record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics, absolute_statistics,  \
+            moments, lr, c1, c2, target);                                                                              \
+    }                                                                                                                  \
+    else {                                                                                                             \
+        launch_gradients_degree<D, Fused, false>(streaming, grid, stream, group_order, partials, record_offsets,       \
+            record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics, absolute_statistics,  \
+            moments, lr, c1, c2, target);                                                                              \
+    }
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
-        if (streaming) {
-            gaussian_gradients_kernel<0, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
-                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, moments, lr, c1, c2);
-        }
-        else {
-            gaussian_gradients_kernel<0, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
-                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, moments, lr, c1, c2);
-        }
+        GSN_LAUNCH_DEGREE(0)
         break;
     case 1:
-        if (streaming) {
-            gaussian_gradients_kernel<1, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
-                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, moments, lr, c1, c2);
-        }
-        else {
-            gaussian_gradients_kernel<1, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
-                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, moments, lr, c1, c2);
-        }
+        GSN_LAUNCH_DEGREE(1)
         break;
     case 2:
-        if (streaming) {
-            gaussian_gradients_kernel<2, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
-                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, moments, lr, c1, c2);
-        }
-        else {
-            gaussian_gradients_kernel<2, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
-                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, moments, lr, c1, c2);
-        }
+        GSN_LAUNCH_DEGREE(2)
         break;
     default:
-        if (streaming) {
-            gaussian_gradients_kernel<3, Fused, Fused><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
-                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, moments, lr, c1, c2);
-        }
-        else {
-            gaussian_gradients_kernel<3, Fused, false><<<grid, kGradientBlock, 0, stream>>>(group_order, partials,
-                record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,
-                absolute_statistics, moments, lr, c1, c2);
-        }
+        GSN_LAUNCH_DEGREE(3)
         break;
     }
+#undef GSN_LAUNCH_DEGREE
     return check_launch("gaussian_gradients_kernel");
 }
 
@@ -799,6 +841,60 @@ extern "C" int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camer
     return launch_gradients<true>(context, cam, count, record_count, sh_degree, group_order, partials,
         record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, moments,
         position_learning_rate, first_correction, second_correction);
+}
+
+extern "C" int gsn_gaussian_gradients_adam_project(gsn_context* context, const gsn_camera* camera, int count,
+    int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets, int record_count,
+    const int* record_offsets, const int* tile_rects, const float* means, const float* conic_opacities,
+    const float* colors, const float* image, const float* image_gradients, float* moments,
+    float position_learning_rate, float first_correction, float second_correction, float* statistics,
+    float* absolute_statistics, const gsn_camera* next_camera, float* next_means, float* next_conic_opacities,
+    float* next_colors, float* next_depths, int* next_tile_rects, int* next_record_offsets) {
+    using namespace gsn;
+    GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
+        record_offsets, tile_rects, means, conic_opacities, colors, image, image_gradients, statistics,
+        absolute_statistics));
+    if (!moments || !(position_learning_rate >= 0.0f) || !std::isfinite(position_learning_rate) || !next_camera ||
+        !next_record_offsets || next_record_offsets == record_offsets ||
+        (count > 0 && (!next_means || !next_conic_opacities || !next_colors || !next_depths || !next_tile_rects))) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_gaussian_gradients_adam_project: invalid arguments");
+    }
+    if (next_camera->width <= 0 || next_camera->height <= 0 || next_camera->tile_count_x <= 0 ||
+        next_camera->tile_count_y <= 0) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_gaussian_gradients_adam_project: invalid next camera");
+    }
+    Camera cam;
+    memcpy(&cam, camera, sizeof(cam));
+    float* partials = static_cast<float*>(
+        scratch(context, kSlotPartials, static_cast<size_t>(record_count > 0 ? record_count : 1) * kPartialFloats * 4));
+    if (!partials) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    const uint8_t* flags = nullptr;
+    uint8_t epoch = 0;
+    const int* group_order = nullptr;
+    if (record_count > 0) {
+        GSN_TRY(launch_splat_backward(context, cam, count, tile_gaussian_ids, tile_offsets, means, conic_opacities,
+            colors, tile_rects, record_offsets, image, image_gradients, partials, record_count, &flags, &epoch,
+            &group_order));
+    }
+    // From here on the next view's outputs are the context's last projection.
+    GSN_TRY(record_projection(context, count, next_means, next_conic_opacities, next_colors, next_depths,
+        next_tile_rects, next_record_offsets));
+    if (count == 0) {
+        return check_hip(hipMemsetAsync(next_record_offsets, 0, sizeof(int), context->stream), "hipMemsetAsync");
+    }
+    NextProjection next;
+    memcpy(&next.camera, next_camera, sizeof(next.camera));
+    next.means = reinterpret_cast<float2*>(next_means);
+    next.conic_opacities = reinterpret_cast<float4*>(next_conic_opacities);
+    next.colors = next_colors;
+    next.depths = next_depths;
+    next.tile_rects = reinterpret_cast<int4*>(next_tile_rects);
+    next.record_counts = next_record_offsets;
+    return launch_gradients<true>(context, cam, count, record_count, sh_degree, group_order, partials,
+        record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, moments,
+        position_learning_rate, first_correction, second_correction, &next);
 }
 
 extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_camera* camera, int count,

@@ -209,6 +209,11 @@ struct StoredState {
 // before all its pixels are done get a partial; for each of them the backward also stores the
 // step's epoch into record_flags[slot] (*flags, *epoch are returned), so the per-Gaussian pass
 // reads exactly the partials written in this step.
+// Makes these buffers the context's last projection (project.hip): the tile-list build, the depth
+// order queued by the record scan and the backward prefetch refer to it.
+int record_projection(gsn_context* context, int count, float* means, float* conic_opacities, float* colors,
+    float* depths, int* tile_rects, int* record_offsets);
+
 // Queues the backward's splat copy and schedules of the last projection on the auxiliary stream
 // (an armed gsn_render_tiles; raster_backward.hip).
 int queue_backward_prefetch(gsn_context* context, const Camera& cam, const int64_t* tile_offsets);

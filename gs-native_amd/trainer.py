@@ -19,13 +19,14 @@ import numpy as np
 
 if __package__:
     from . import CapacityBuffer, DeviceBuffer, Moments, Projection, adam_corrections, call, gaussian_gradients
-    from . import gaussian_gradients_adam, image_gradients, render, _p
+    from . import gaussian_gradients_adam, gaussian_gradients_adam_project, image_gradients, render, render_projected, _p
 else:  # loaded by path
     _pkg = sys.modules["gs_native_amd"]
     DeviceBuffer, Projection, adam_corrections, call = _pkg.DeviceBuffer, _pkg.Projection, _pkg.adam_corrections, _pkg.call
     gaussian_gradients, gaussian_gradients_adam = _pkg.gaussian_gradients, _pkg.gaussian_gradients_adam
     image_gradients, render, _p, Moments = _pkg.image_gradients, _pkg.render, _pkg._p, _pkg.Moments
     CapacityBuffer = _pkg.CapacityBuffer
+    gaussian_gradients_adam_project, render_projected = _pkg.gaussian_gradients_adam_project, _pkg.render_projected
 
 
 FEATURES = 16
@@ -116,6 +117,12 @@ class Trainer:
         self.sh_degree = 0
         self.projection = None
         self.overlap_backward = True
+        # Fused projection: the gradient pass also projects the updated parameters for the next
+        # camera into spare_projection; next_view = (camera, count, SH degree) it was made for, or
+        # None when parameters changed since (density control, opacity caps).
+        self.fuse_projection = True
+        self.spare_projection = None
+        self.next_view = None
         self.rng = np.random.default_rng(seed)
         self.order = np.arange(len(cameras))
         self.cursor = len(cameras)
@@ -149,6 +156,14 @@ class Trainer:
                                     + math.log(np.float32(0.0000016)) * fraction))
         return float(np.float32(self.scene_extent) * value)
 
+    def peek_camera(self) -> int:
+        """The camera the next next_camera() returns (when a cycle ends the order is reshuffled now
+        instead of then: the generator serves only these shuffles, so the sequence is the same)."""
+        if self.cursor >= len(self.order):
+            self.rng.shuffle(self.order)
+            self.cursor = 0
+        return int(self.order[self.cursor])
+
     def next_camera(self) -> int:
         if self.cursor >= len(self.order):
             self.rng.shuffle(self.order)
@@ -168,12 +183,25 @@ class Trainer:
             # The backward's preparation then runs on a second stream beside the forward raster
             # and the image gradient; results are identical either way.
             self.context.prefetch_backward()
-        self.projection = render(self.context, self.params, self.count, camera, self.background, self.image,
-                                 self.sh_degree, self.projection)
+        if self.next_view == (index, self.count, self.sh_degree):
+            # The previous gradient pass projected the current parameters for this camera.
+            self.projection, self.spare_projection = self.spare_projection, self.projection
+            render_projected(self.context, self.projection, self.background, self.image)
+        else:
+            self.projection = render(self.context, self.params, self.count, camera, self.background, self.image,
+                                     self.sh_degree, self.projection)
+        self.next_view = None
         image_gradients(self.context, camera, self.image, self.targets[index], self.image_grad)
         self.adam_step += 1
         c1, c2 = adam_corrections(self.adam_step)
-        if self.fused:
+        if self.fused and self.fuse_projection and min(3, (iteration + 1) // 1000) == self.sh_degree:
+            next_index = self.peek_camera()
+            self.spare_projection = gaussian_gradients_adam_project(
+                self.context, self.projection, self.params, self.sh_degree, self.image, self.image_grad,
+                self.moments, lr, c1, c2, self.statistics, self.absolute_statistics, self.cameras[next_index],
+                self.spare_projection)
+            self.next_view = (next_index, self.count, self.sh_degree)
+        elif self.fused:
             gaussian_gradients_adam(self.context, self.projection, self.params, self.sh_degree, self.image,
                                     self.image_grad, self.moments, lr, c1, c2, self.statistics,
                                     self.absolute_statistics)
@@ -201,6 +229,7 @@ class Trainer:
 
     # -- density -------------------------------------------------------------------------------
     def reset_opacity(self, cap: float) -> None:
+        self.next_view = None
         if cap < 0.5:
             # Opacities collapse: outcomes of the pending density step are not measurable.
             self.pending = None
@@ -210,6 +239,7 @@ class Trainer:
 
     def control_density(self, gradient_threshold=0.0002, percent_dense=0.01, min_opacity=0.005,
                         absolute_threshold=0.0008) -> tuple[int, int]:
+        self.next_view = None
         count = self.count
         max_screen = 20.0 if self.completed > 3000 else 0.0
         decisions = self._stores["decisions"].get((count + 1, 4))

@@ -250,6 +250,24 @@ int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, 
     const float* image_gradients, float* moments, float position_learning_rate,
     float first_correction, float second_correction, float* statistics, float* absolute_statistics);
 
+/* gsn_gaussian_gradients_adam followed by gsn_project_gaussians of the updated parameters for
+ * next_camera (same SH degree), fused: the pass projects each Gaussian from the values it has
+ * just written, so the next iteration's projection reads no parameter rows. The next_* buffers
+ * receive exactly what gsn_project_gaussians would write (record counts in next_record_offsets,
+ * ready for gsn_scan_record_offsets) and become the context's last projection.
+ * next_record_offsets must not be record_offsets; the other next_* buffers may be the current
+ * projection's (the gradient pass does not read them). Replaces Solver::Iterate
+ * (solver.cu:702-720) and the projection of the following Renderer::Render (renderer.cu:374-444)
+ * when nothing changes the parameters in between (density control and opacity caps do). */
+int gsn_gaussian_gradients_adam_project(gsn_context* context, const gsn_camera* camera, int count,
+    int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
+    int record_count, const int* record_offsets, const int* tile_rects, const float* means,
+    const float* conic_opacities, const float* colors, const float* image,
+    const float* image_gradients, float* moments, float position_learning_rate,
+    float first_correction, float second_correction, float* statistics, float* absolute_statistics,
+    const gsn_camera* next_camera, float* next_means, float* next_conic_opacities, float* next_colors,
+    float* next_depths, int* next_tile_rects, int* next_record_offsets);
+
 /* Dense Adam over all 58*N entries with the reference's per-group learning rates.
  * Replaces UpdateGaussiansKernel (solver.cu:651-689, 901-929). */
 int gsn_adam_update(gsn_context* context, int count, const float* gradients, float* params,
