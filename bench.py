@@ -102,7 +102,9 @@ def algorithmic_bytes(*, n: float, v: float, r: float, p: int, t: int, k: int = 
         # with the next view's projection fused in: its outputs (record count for every Gaussian;
         # mean, conic + opacity, colour, depth, rectangle for the visible ones); it reads nothing more
         "gaussian_gradients_projected": 1416 * n + 56 * v + r,
-        "gaussian_gradients_unfused": 484 * n + r,  # params read, gradients written, statistics, offsets, flags
+        # gradients written, statistics, offsets, flags; the parameter rows are read only for the
+        # Gaussians that received a partial (232 bytes each, not counted: a lower bound)
+        "gaussian_gradients_unfused": 252 * n + r,
         "adam": 1624 * n,                           # params, moments read + written, gradients read
         "accumulate": 24 * n,
     }
@@ -348,7 +350,9 @@ def main() -> int:
         }
         per_stage_note = (f"iterations {stage_first}-{trainer.completed} with an event pair around every stage; the "
                           f"gradient pass runs unfused there (writes the 58-row gradients, then the row-major Adam "
-                          f"kernel: bit-identical results) so that Adam has its own time")
+                          f"kernel: bit-identical results) so that Adam has its own time; in that form it reads the "
+                          f"parameter rows only of the Gaussians that received a partial, which its algorithmic "
+                          f"bytes (252N + R) leave out, and the projection runs as its own kernel")
         log(f"stage pass done: iterations {stage_first}-{trainer.completed}, {pairs:.3e} raster pairs per view")
     else:
         per_stage_note = "no stage pass"
