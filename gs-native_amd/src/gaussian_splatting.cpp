@@ -187,6 +187,7 @@ bool GaussianSplatting::Initialize(const Scene& scene) {
     cycle_position_ = cycle_.size();  // the first Train iteration shuffles
     completed_ = 0;
     sh_degree_ = 0;
+    next_view_ = -1;
 
     const int count = static_cast<int>(points.positions_.size());
     std::vector<float> mean_distance2 = ComputeMeanDist2Cuda(scene_.point_cloud_.positions_);
@@ -230,9 +231,36 @@ bool GaussianSplatting::Step() {
         GS_NATIVE_ERROR("arming the backward prefetch failed: %s", gsn_last_error());
         return false;
     }
-    if (!renderer_.Render(gaussians_, camera, background_, image_, sh_degree_) ||
-        !solver_.Iterate(camera, renderer_, image_, targets_[static_cast<size_t>(view)], gaussians_,
-            plan.position_learning_rate, sh_degree_)) {
+    bool ok = true;
+    if (next_view_ == view && next_count_ == gaussians_.gaussian_count_ && next_degree_ == sh_degree_) {
+        // The previous gradient pass projected the current parameters for this camera.
+        current_ = 1 - current_;
+        ok = renderer().RenderProjected(gaussians_, camera, background_, image_);
+    }
+    else {
+        ok = renderer().Render(gaussians_, camera, background_, image_, sh_degree_);
+    }
+    next_view_ = -1;
+    const uint8_t* target = targets_[static_cast<size_t>(view)];
+    // The next camera is known unless this pass over the views ends here (Train reseeds the
+    // shuffle, so the next pass's order is drawn only when it starts).
+    const bool fuse = cycle_position_ + 1 < cycle_.size() &&
+                      schedule::StepFor(iteration + 1, scene_extent_).sh_degree == sh_degree_;
+    if (ok && fuse) {
+        const int next = cycle_[cycle_position_ + 1];
+        const CameraParams next_camera = scene_.train_cameras_[static_cast<size_t>(next)].GetCameraParams();
+        ok = solver_.IterateAndProject(camera, renderer(), image_, target, gaussians_, plan.position_learning_rate,
+            sh_degree_, next_camera, renderers_[1 - current_]);
+        if (ok) {
+            next_view_ = next;
+            next_count_ = gaussians_.gaussian_count_;
+            next_degree_ = sh_degree_;
+        }
+    }
+    else if (ok) {
+        ok = solver_.Iterate(camera, renderer(), image_, target, gaussians_, plan.position_learning_rate, sh_degree_);
+    }
+    if (!ok) {
         GS_NATIVE_ERROR("training iteration %d on view %d stopped: %s", iteration, view, gsn_last_error());
         return false;
     }
@@ -248,10 +276,13 @@ bool GaussianSplatting::Upkeep() {
         const float* statistics = solver_.GetProjectionStatistics();
         const float* absolute = solver_.GetAbsoluteProjectionStatistics();
         if (!density_.AccumulateStatistics(statistics, absolute, count) ||
-            !density_.AccumulateFeatures(statistics, absolute, renderer_.GetDepths(),
-                reinterpret_cast<const float*>(renderer_.GetConicOpacities()), count)) {
+            !density_.AccumulateFeatures(statistics, absolute, renderer().GetDepths(),
+                reinterpret_cast<const float*>(renderer().GetConicOpacities()), count)) {
             return false;
         }
+    }
+    if (upkeep.control_density || upkeep.opacity_cap > 0.0f) {
+        next_view_ = -1;  // the parameters change: the projection made for the next camera is stale
     }
     if (upkeep.control_density) {
         int changes[2] = {0, 0};
@@ -297,7 +328,7 @@ bool GaussianSplatting::Render(const Camera& source, ImageF& image) {
     }
     const size_t pixels = static_cast<size_t>(width) * height;
     const CameraParams camera = source.GetCameraParams();
-    if (!EnsureImageCapacity(pixels) || !renderer_.Render(gaussians_, camera, background_, image_, sh_degree_) ||
+    if (!EnsureImageCapacity(pixels) || !renderer().Render(gaussians_, camera, background_, image_, sh_degree_) ||
         !Check(gsn_context_synchronize(DefaultContext()), "gsn_context_synchronize")) {
         return false;
     }

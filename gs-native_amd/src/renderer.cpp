@@ -53,13 +53,28 @@ bool Renderer::Render(
     return RenderTiles(camera, background, image);
 }
 
-bool Renderer::ProjectGaussians(const GaussianBuffers& gaussians, const CameraParams& camera, int sh_degree) {
-    int count = gaussians.gaussian_count_;
-    gsn_context* context = DefaultContext();
-    if (!context || count < 0 || (count > 0 && !gaussians.gaussians_)) {
+bool Renderer::RenderProjected(
+    const GaussianBuffers& gaussians,
+    const CameraParams& camera,
+    const float* background,
+    float* image) {
+    return BuildTileLists(gaussians, camera) && RenderTiles(camera, background, image);
+}
+
+bool Renderer::ProjectionBuffers(int count, ProjectionTargets* targets) {
+    if (!targets || count < 0 || !EnsureGaussianCapacity(count)) {
         return false;
     }
-    sh_degree = std::clamp(sh_degree, 0, 3);
+    targets->means = reinterpret_cast<float*>(means_);
+    targets->conic_opacities = reinterpret_cast<float*>(conic_opacities_);
+    targets->colors = reinterpret_cast<float*>(colors_);
+    targets->depths = depths_;
+    targets->tile_rects = reinterpret_cast<int*>(tile_rects_);
+    targets->record_offsets = record_offsets_;
+    return true;
+}
+
+bool Renderer::EnsureGaussianCapacity(int count) {
     if (count > gaussian_capacity_ || !record_offsets_) {
         ReleaseGaussianBuffers();
         size_t n = static_cast<size_t>(count);
@@ -75,6 +90,16 @@ bool Renderer::ProjectGaussians(const GaussianBuffers& gaussians, const CameraPa
         }
         gaussian_capacity_ = count;
     }
+    return true;
+}
+
+bool Renderer::ProjectGaussians(const GaussianBuffers& gaussians, const CameraParams& camera, int sh_degree) {
+    int count = gaussians.gaussian_count_;
+    gsn_context* context = DefaultContext();
+    if (!context || count < 0 || (count > 0 && !gaussians.gaussians_) || !EnsureGaussianCapacity(count)) {
+        return false;
+    }
+    sh_degree = std::clamp(sh_degree, 0, 3);
     return Check(gsn_project_gaussians(context, gaussians.gaussians_, count, AsCamera(&camera), sh_degree,
                      reinterpret_cast<float*>(means_), reinterpret_cast<float*>(conic_opacities_),
                      reinterpret_cast<float*>(colors_), depths_, reinterpret_cast<int*>(tile_rects_), record_offsets_),

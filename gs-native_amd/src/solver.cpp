@@ -26,6 +26,36 @@ bool Solver::Iterate(
     GaussianBuffers& gaussians,
     float position_learning_rate,
     int sh_degree) {
+    return Step(camera, renderer, image, target, gaussians, position_learning_rate, sh_degree, nullptr, nullptr);
+}
+
+bool Solver::IterateAndProject(
+    const CameraParams& camera,
+    const Renderer& renderer,
+    const float* image,
+    const uint8_t* target,
+    GaussianBuffers& gaussians,
+    float position_learning_rate,
+    int sh_degree,
+    const CameraParams& next_camera,
+    Renderer& next_renderer) {
+    if (&next_renderer == &renderer) {
+        return false;
+    }
+    return Step(camera, renderer, image, target, gaussians, position_learning_rate, sh_degree, &next_camera,
+        &next_renderer);
+}
+
+bool Solver::Step(
+    const CameraParams& camera,
+    const Renderer& renderer,
+    const float* image,
+    const uint8_t* target,
+    GaussianBuffers& gaussians,
+    float position_learning_rate,
+    int sh_degree,
+    const CameraParams* next_camera,
+    Renderer* next_renderer) {
     gsn_context* context = DefaultContext();
     if (!context || !ComputeImageGradients(camera, image, target)) {
         return false;
@@ -53,6 +83,25 @@ bool Solver::Iterate(
     uint64_t iteration = iteration_ + 1;
     float first_correction = static_cast<float>(1.0 / (1.0 - std::pow(0.9, static_cast<double>(iteration))));
     float second_correction = static_cast<float>(1.0 / (1.0 - std::pow(0.999, static_cast<double>(iteration))));
+    if (next_renderer) {
+        ProjectionTargets next;
+        bool ok = next_renderer->ProjectionBuffers(count, &next) &&
+                  Check(gsn_gaussian_gradients_adam_project(context, AsCamera(&camera), count,
+                            std::clamp(sh_degree, 0, 3), gaussians.gaussians_, renderer.GetTileGaussianIds(),
+                            renderer.GetTileGaussianOffsets(), renderer.GetRecordCount(), renderer.GetRecordOffsets(),
+                            reinterpret_cast<const int*>(renderer.GetTileRects()),
+                            reinterpret_cast<const float*>(renderer.GetMeans()),
+                            reinterpret_cast<const float*>(renderer.GetConicOpacities()),
+                            reinterpret_cast<const float*>(renderer.GetColors()), image, image_gradients_, moments_,
+                            position_learning_rate, first_correction, second_correction, projection_statistics_,
+                            absolute_projection_statistics_, AsCamera(next_camera), next.means, next.conic_opacities,
+                            next.colors, next.depths, next.tile_rects, next.record_offsets),
+                      "gsn_gaussian_gradients_adam_project");
+        if (ok) {
+            iteration_ = iteration;
+        }
+        return ok;
+    }
     bool ok = Check(gsn_gaussian_gradients_adam(context, AsCamera(&camera), count, std::clamp(sh_degree, 0, 3),
                         gaussians.gaussians_, renderer.GetTileGaussianIds(), renderer.GetTileGaussianOffsets(),
                         renderer.GetRecordCount(), renderer.GetRecordOffsets(),

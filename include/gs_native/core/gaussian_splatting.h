@@ -52,7 +52,17 @@ private:
     Scene scene_;
     GaussianBuffers gaussians_;
 
-    Renderer renderer_;
+    // Two renderers: the gradient pass projects the updated parameters for the next camera into
+    // the spare one, which then renders that view (Solver::IterateAndProject). next_view_ is the
+    // (camera, count, SH degree) of that projection, or camera -1 when parameters changed since.
+    Renderer renderers_[2];
+    int current_ = 0;
+    int next_view_ = -1;
+    int next_count_ = 0;
+    int next_degree_ = 0;
+    Renderer& renderer() {
+        return renderers_[current_];
+    }
     Solver solver_;
     DensityController density_;
 

@@ -13,6 +13,16 @@
 
 namespace gs_native {
 
+// The per-Gaussian outputs of a projection (device pointers; see Renderer::ProjectionBuffers).
+struct ProjectionTargets {
+    float* means = nullptr;
+    float* conic_opacities = nullptr;
+    float* colors = nullptr;
+    float* depths = nullptr;
+    int* tile_rects = nullptr;
+    int* record_offsets = nullptr;
+};
+
 class Renderer {
 public:
     Renderer() = default;
@@ -27,6 +37,16 @@ public:
         const float* background,
         float* image,
         int sh_degree);
+
+    // Additions for the fused training step: the per-Gaussian projection buffers for `count`
+    // Gaussians (grown if needed) for a producer other than Render (Solver::IterateAndProject),
+    // and Render from a projection already written there (tile lists and rasteriser only).
+    bool ProjectionBuffers(int count, ProjectionTargets* targets);
+    bool RenderProjected(
+        const GaussianBuffers& gaussians,
+        const CameraParams& camera,
+        const float* background,
+        float* image);
 
     const float2* GetMeans() const {
         return means_;
@@ -76,6 +96,7 @@ public:
 
 private:
     bool ProjectGaussians(const GaussianBuffers& gaussians, const CameraParams& camera, int sh_degree);
+    bool EnsureGaussianCapacity(int count);
     bool BuildTileLists(const GaussianBuffers& gaussians, const CameraParams& camera);
     bool RenderTiles(const CameraParams& camera, const float* background, float* image);
     void ReleaseGaussianBuffers();

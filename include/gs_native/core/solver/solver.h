@@ -27,6 +27,21 @@ public:
         float position_learning_rate,
         int sh_degree);
 
+    // Addition: Iterate fused with the projection of the updated parameters for next_camera into
+    // next_renderer's buffers (gsn_gaussian_gradients_adam_project); next_renderer then renders
+    // that view with RenderProjected, provided nothing changes the parameters or the SH degree in
+    // between. next_renderer must not be renderer.
+    bool IterateAndProject(
+        const CameraParams& camera,
+        const Renderer& renderer,
+        const float* image,
+        const uint8_t* target,
+        GaussianBuffers& gaussians,
+        float position_learning_rate,
+        int sh_degree,
+        const CameraParams& next_camera,
+        Renderer& next_renderer);
+
     bool ResetOpacityHistory();
 
     bool RemapGaussians(
@@ -42,6 +57,16 @@ public:
     }
 
 private:
+    bool Step(
+        const CameraParams& camera,
+        const Renderer& renderer,
+        const float* image,
+        const uint8_t* target,
+        GaussianBuffers& gaussians,
+        float position_learning_rate,
+        int sh_degree,
+        const CameraParams* next_camera,
+        Renderer* next_renderer);
     bool ComputeImageGradients(const CameraParams& camera, const float* image, const uint8_t* target);
 
     float* image_gradients_ = nullptr;

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
@@ -418,6 +419,51 @@ void SolverOpacityHistoryReset() {
     EXPECT(std::abs(std::abs(actual[kOpacityIndex] - values[kOpacityIndex]) - 0.025 * factor) < 2e-6);
 }
 
+// Solver::IterateAndProject (the next view's projection fused into the gradient pass) against
+// Iterate followed by Render of that view: parameters and the rendered image bit for bit, at SH
+// degree 3 on 300 Gaussians.
+void SolverIterateAndProjectMatchesIterateThenRender() {
+    CameraParams camera = MakeCamera(40, 36);
+    CameraParams next_camera = MakeCamera(40, 36);
+    next_camera.t_c_w_[0] = 0.05f;
+    next_camera.fx_ = 22.0f;
+    auto gaussians = MakeGaussians(300);
+    DeviceVector<uint8_t> target(std::vector<uint8_t>(40 * 36 * 3, 150));
+    float background[3] = {0.1f, 0.2f, 0.3f};
+    std::vector<float> results[2];
+    std::vector<float> images[2];
+    for (int fused = 0; fused < 2; ++fused) {
+        DeviceVector<float> parameters(PackGaussians(gaussians));
+        GaussianBuffers buffers{parameters.data, 300};
+        DeviceVector<float> image(40 * 36 * 3);
+        DeviceVector<float> next_image(40 * 36 * 3);
+        Renderer renderer;
+        Renderer next_renderer;
+        Solver solver;
+        EXPECT(renderer.Render(buffers, camera, background, image.data, 3));
+        if (fused) {
+            EXPECT(!solver.IterateAndProject(camera, renderer, image.data, target.data, buffers, 0.0001f, 3, next_camera,
+                renderer));
+            EXPECT(solver.IterateAndProject(camera, renderer, image.data, target.data, buffers, 0.0001f, 3, next_camera,
+                next_renderer));
+            EXPECT(next_renderer.RenderProjected(buffers, next_camera, background, next_image.data));
+        }
+        else {
+            EXPECT(solver.Iterate(camera, renderer, image.data, target.data, buffers, 0.0001f, 3));
+            EXPECT(next_renderer.Render(buffers, next_camera, background, next_image.data, 3));
+        }
+        results[fused] = parameters.Download();
+        images[fused] = next_image.Download();
+    }
+    EXPECT(std::memcmp(results[0].data(), results[1].data(), results[0].size() * sizeof(float)) == 0);
+    EXPECT(std::memcmp(images[0].data(), images[1].data(), images[0].size() * sizeof(float)) == 0);
+    double total = 0.0;
+    for (float v : images[1]) {
+        total += v;
+    }
+    EXPECT(total > 0.0);
+}
+
 // knn_test.cu:44-53 through the templated host API.
 void KnnPreservesOrder() {
     std::vector<Vector3f> points = {MakeVector3f(100, 0, 0), MakeVector3f(0, 0, 0), MakeVector3f(1, 0, 0),
@@ -438,6 +484,7 @@ void KnnPreservesOrder() {
 int main(int argc, char** argv) {
     std::vector<std::pair<std::string, std::function<void()>>> tests = {
         {"InitializesIteratesSavesAndReloads", InitializesIteratesSavesAndReloads},
+        {"SolverIterateAndProjectMatchesIterateThenRender", SolverIterateAndProjectMatchesIterateThenRender},
         {"CrossesDensityControlBoundary", CrossesDensityControlBoundary},
         {"RejectsEmptySceneAndSupportsZeroIterations", RejectsEmptySceneAndSupportsZeroIterations},
         {"DensityControllerPrunesAndSwaps", DensityControllerPrunesAndSwaps},
