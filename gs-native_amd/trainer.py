@@ -75,7 +75,15 @@ class Trainer:
     outcome label) samples; it labels each density step at the next one. fused (attribute): True
     runs the gradient pass with Adam fused in (no gradient buffer); False writes the [58][N]
     gradients and runs the separate row-major Adam kernel, with bit-identical results (used to
-    time Adam on its own)."""
+    time Adam on its own). overlap_backward and fuse_projection (attributes, default True) run the
+    backward's preparation on the context's second stream and project the next camera's view in
+    the gradient pass; both give bit-identical results. Code that changes `params` outside the
+    trainer's own methods must call invalidate_projection() first."""
+
+    def invalidate_projection(self) -> None:
+        """Drops the projection the last gradient pass made for the next camera (parameters are
+        about to change): the next iteration projects again."""
+        self.next_view = None
 
     def __init__(self, context, params: np.ndarray, cameras: list, targets: list, scene_extent: float,
                  background=(0.0, 0.0, 0.0), max_count: int | None = None, completed: int = 0, seed: int = 0,
@@ -229,7 +237,7 @@ class Trainer:
 
     # -- density -------------------------------------------------------------------------------
     def reset_opacity(self, cap: float) -> None:
-        self.next_view = None
+        self.invalidate_projection()
         if cap < 0.5:
             # Opacities collapse: outcomes of the pending density step are not measurable.
             self.pending = None
@@ -239,7 +247,7 @@ class Trainer:
 
     def control_density(self, gradient_threshold=0.0002, percent_dense=0.01, min_opacity=0.005,
                         absolute_threshold=0.0008) -> tuple[int, int]:
-        self.next_view = None
+        self.invalidate_projection()
         count = self.count
         max_screen = 20.0 if self.completed > 3000 else 0.0
         decisions = self._stores["decisions"].get((count + 1, 4))
