@@ -214,8 +214,12 @@ struct StoredState {
 int record_projection(gsn_context* context, int count, float* means, float* conic_opacities, float* colors,
     float* depths, int* tile_rects, int* record_offsets);
 
-// Queues the backward's splat copy and schedules of the last projection on the auxiliary stream
-// (an armed gsn_render_tiles; raster_backward.hip).
+// The backward prefetch of an armed gsn_render_tiles (raster_backward.hip), in two parts so that
+// the forward kernel is launched in between: fork_backward_prefetch joins earlier queued work,
+// allocates the buffers and records the fork event on the main stream (*forked: whether it did);
+// queue_backward_prefetch then queues the splat copy and schedules of the last projection on the
+// auxiliary stream behind that event.
+int fork_backward_prefetch(gsn_context* context, const Camera& cam, bool* forked);
 int queue_backward_prefetch(gsn_context* context, const Camera& cam, const int64_t* tile_offsets);
 // Makes the main stream wait for queued prefetch work (nothing if none is pending).
 int join_backward_prefetch(gsn_context* context);

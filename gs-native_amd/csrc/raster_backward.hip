@@ -563,7 +563,8 @@ int join_backward_prefetch(gsn_context* context) {
     return check_hip(hipStreamWaitEvent(context->stream, context->join_event, 0), "hipStreamWaitEvent");
 }
 
-int queue_backward_prefetch(gsn_context* context, const Camera& cam, const int64_t* tile_offsets) {
+int fork_backward_prefetch(gsn_context* context, const Camera& cam, bool* forked) {
+    *forked = false;
     GSN_TRY(join_backward_prefetch(context));
     const int count = context->projected_gaussians;
     if (count <= 0 || !context->aux_stream || cam.tile_count_x > 0xffff || cam.tile_count_y > 0x7fff) {
@@ -573,6 +574,14 @@ int queue_backward_prefetch(gsn_context* context, const Camera& cam, const int64
     BackwardBuffers buffers;
     GSN_TRY(backward_buffers(context, cam, count, &buffers));
     GSN_HIP(hipEventRecord(context->fork_event, context->stream));
+    *forked = true;
+    return GSN_OK;
+}
+
+int queue_backward_prefetch(gsn_context* context, const Camera& cam, const int64_t* tile_offsets) {
+    const int count = context->projected_gaussians;
+    BackwardBuffers buffers;
+    GSN_TRY(backward_buffers(context, cam, count, &buffers));  // allocated by the fork: no growth here
     GSN_HIP(hipStreamWaitEvent(context->aux_stream, context->fork_event, 0));
     GSN_TRY(launch_backward_preparation(context->aux_stream, cam, count, tile_offsets, context->projected_means,
         context->projected_conics, context->projected_colors, context->projected_rects,

@@ -143,6 +143,10 @@ __global__ __launch_bounds__(64) void raster_forward_kernel(const int* __restric
 namespace gsn {
 namespace {
 
+int launch_forward(gsn_context* context, const Camera& cam, const int* tile_gaussian_ids,
+    const int64_t* tile_offsets, const float* means, const float* conic_opacities, const float* colors,
+    const float* background, float* image, float* final_transmittance, int* last_contributor, unsigned tiles);
+
 int render_tiles(gsn_context* context, const gsn_camera* camera, const int* tile_gaussian_ids,
     const int64_t* tile_offsets, const float* means, const float* conic_opacities, const float* colors,
     const float* background, float* image, float* final_transmittance, int* last_contributor) {
@@ -156,16 +160,27 @@ int render_tiles(gsn_context* context, const gsn_camera* camera, const int* tile
     }
     Camera cam;
     memcpy(&cam, camera, sizeof(cam));
-    unsigned long long* pair_count = context->count_raster_pairs ? context->raster_pairs : nullptr;
+    bool forked = false;
     if (context->prefetch_armed) {
         // Armed by gsn_context_prefetch_backward: the backward's preparation of this projection
-        // runs on the auxiliary stream beside this kernel and the image gradient.
+        // runs on the auxiliary stream beside this kernel and the image gradient. The fork point
+        // is recorded before the kernel and the auxiliary launches are issued after it, so the
+        // host's work for them does not delay the forward.
         context->prefetch_armed = false;
         if (means == context->projected_means && conic_opacities == context->projected_conics &&
             colors == context->projected_colors) {
-            GSN_TRY(queue_backward_prefetch(context, cam, tile_offsets));
+            GSN_TRY(fork_backward_prefetch(context, cam, &forked));
         }
     }
+    GSN_TRY(launch_forward(context, cam, tile_gaussian_ids, tile_offsets, means, conic_opacities, colors, background,
+        image, final_transmittance, last_contributor, static_cast<unsigned>(tiles)));
+    return forked ? queue_backward_prefetch(context, cam, tile_offsets) : GSN_OK;
+}
+
+int launch_forward(gsn_context* context, const Camera& cam, const int* tile_gaussian_ids,
+    const int64_t* tile_offsets, const float* means, const float* conic_opacities, const float* colors,
+    const float* background, float* image, float* final_transmittance, int* last_contributor, unsigned tiles) {
+    unsigned long long* pair_count = context->count_raster_pairs ? context->raster_pairs : nullptr;
     StageScope scope(context, kStageRasterForward);
     const unsigned grid = static_cast<unsigned>(tiles);
     const float2* m = reinterpret_cast<const float2*>(means);

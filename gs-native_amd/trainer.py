@@ -110,6 +110,14 @@ class Trainer:
         self.params = self._stores["params_a"].get((58, self.count))
         self.params.upload(np.ascontiguousarray(params, np.float32))
         self.moments = Moments(self.count, self._stores["moments_a"])
+        # The buffers density control writes into (spare parameters and moments, decisions,
+        # offsets, history) are allocated now, with the same 25% headroom, instead of at the first
+        # density step: a step then allocates nothing unless the count outgrows the headroom.
+        self._stores["params_b"].get((58, self.count))
+        Moments(self.count, self._stores["moments_b"], zero=False)
+        for name in ("decisions", "offsets"):
+            self._stores[name].get((self.count + 1, 4))
+        self._stores["history"].get((max(self.count, 1),))
         self._allocate_statistics(self.count)
         self.cameras = cameras
         self.targets = [t if isinstance(t, DeviceBuffer) else DeviceBuffer.from_numpy(np.ascontiguousarray(t, np.uint8))
