@@ -2,10 +2,13 @@
 //
 // Each pass (<= 8 digit bits) runs three kernels over 4096-element tiles:
 //   1. radix_histogram: per-tile digit histogram in LDS, written bin-major [bin][tile].
-//   2. exclusive scan of the histogram table (scan.hip) -> global base of every (bin, tile).
-//   3. radix_scatter: stable in-tile ranking with wave ballots (digit match masks), local
-//      reordering through LDS, then a write-out in which consecutive lanes store consecutive
-//      positions of a digit run.
+//   2. radix_digit_scan: one workgroup per digit scans its row of the table in place (the
+//      digit's start inside every tile's share) and writes the row total.
+//   3. radix_scatter: adds the exclusive prefix of the row totals (the digit's global start),
+//      then stable in-tile ranking with wave ballots (digit match masks), local reordering
+//      through LDS, and a write-out in which consecutive lanes store consecutive positions of a
+//      digit run.
+// The row scan replaces a device-wide scan of the whole table (two launches per pass).
 // Stability: a tile is split into eight 512-element wave chunks processed in index order, and
 // ranks are offset by the digit counts of earlier waves, so equal digits keep input order.
 #include "gsn_internal.h"
@@ -53,14 +56,72 @@ __global__ __launch_bounds__(kSortThreads) void radix_histogram_kernel(const uin
     }
 }
 
+// Exclusive scan of digit blockIdx.x's row of the [bin][tile] table, in place, in chunks of
+// kSortTile entries (eight consecutive entries per thread); the row total goes to totals.
+__global__ __launch_bounds__(kSortThreads) void radix_digit_scan_kernel(int* histogram, int tiles, int* totals) {
+    __shared__ int s_wave[kWaves];
+    int* row = histogram + static_cast<int64_t>(blockIdx.x) * tiles;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    int carry = 0;
+    for (int chunk = 0; chunk < tiles; chunk += kSortTile) {
+        int values[kSortItems];
+        int sum = 0;
+#pragma unroll
+        for (int i = 0; i < kSortItems; ++i) {
+            int index = chunk + t * kSortItems + i;
+            values[i] = index < tiles ? row[index] : 0;
+            sum += values[i];
+        }
+        int inclusive = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int other = __shfl_up(inclusive, d, 64);
+            if (lane >= d) {
+                inclusive += other;
+            }
+        }
+        if (lane == 63) {
+            s_wave[wave] = inclusive;
+        }
+        __syncthreads();
+        int before = 0;
+        int chunk_total = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            int v = s_wave[w];
+            before += w < wave ? v : 0;
+            chunk_total += v;
+        }
+        int running = carry + before + inclusive - sum;
+#pragma unroll
+        for (int i = 0; i < kSortItems; ++i) {
+            int index = chunk + t * kSortItems + i;
+            if (index < tiles) {
+                row[index] = running;
+            }
+            running += values[i];
+        }
+        carry += chunk_total;
+        // s_wave is rewritten by the next chunk.
+        __syncthreads();
+    }
+    if (t == 0) {
+        totals[blockIdx.x] = carry;
+    }
+}
+
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint32_t* keys_in, const uint32_t* values_in,
-    int64_t n, int shift, int bits, int tiles, const int* offsets, uint32_t* keys_out, uint32_t* values_out) {
+    int64_t n, int shift, int bits, int tiles, const int* offsets, const int* totals, uint32_t* keys_out,
+    uint32_t* values_out) {
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_values[kSortTile];
     __shared__ int s_wave_count[kWaves][256];
     __shared__ int s_digit_start[256];
     __shared__ int s_global[256];
     __shared__ int s_scan[256];
+    __shared__ int s_base[256];
 
     const int t = threadIdx.x;
     const int wave = t >> 6;
@@ -115,8 +176,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
     }
     __syncthreads();
 
-    // Per digit: exclusive prefix across waves (in place) and the tile total.
+    // Per digit: exclusive prefix across waves (in place) and the tile total; the digit's start
+    // in this tile's share of the output (its row-scanned table entry) and its row total.
     int digit_total = 0;
+    int row_total = 0;
     if (t < bins) {
         int running = 0;
         for (int w = 0; w < kWaves; ++w) {
@@ -125,22 +188,30 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
             running += c;
         }
         digit_total = running;
+        row_total = totals[t];
         s_global[t] = offsets[static_cast<int64_t>(t) * tiles + blockIdx.x];
     }
     if (t < 256) {
-        s_scan[t] = t < bins ? digit_total : 0;
+        s_scan[t] = digit_total;
+        s_base[t] = row_total;
     }
     __syncthreads();
+    // Inclusive scans over the digits of the in-tile counts and of the row totals.
     for (int offset = 1; offset < 256; offset <<= 1) {
         int other = (t < 256 && t >= offset) ? s_scan[t - offset] : 0;
+        int other_base = (t < 256 && t >= offset) ? s_base[t - offset] : 0;
         __syncthreads();
         if (t < 256) {
             s_scan[t] += other;
+            s_base[t] += other_base;
         }
         __syncthreads();
     }
     if (t < 256) {
-        s_digit_start[t] = s_scan[t] - (t < bins ? digit_total : 0);
+        s_digit_start[t] = s_scan[t] - digit_total;
+    }
+    if (t < bins) {
+        s_global[t] += s_base[t] - row_total;
     }
     __syncthreads();
 
@@ -167,9 +238,11 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
 
 } // namespace
 
+// The [256][tiles] table, followed by 256 row totals.
 int* radix_histogram_buffer(gsn_context* context, int64_t n) {
     int64_t tiles = (n + kSortTile - 1) / kSortTile;
-    return static_cast<int*>(scratch(context, kSlotSortHistogram, static_cast<size_t>(tiles) * 256 * sizeof(int)));
+    return static_cast<int*>(
+        scratch(context, kSlotSortHistogram, (static_cast<size_t>(tiles) + 1) * 256 * sizeof(int)));
 }
 
 // Digit widths: the narrower digits come first (13 tile bits: 6 then 7). The first pass of the
@@ -204,6 +277,7 @@ int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int
     if (!scratch_keys || !scratch_values || !histogram) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
+    int* totals = histogram + static_cast<int64_t>(tiles) * 256;
     uint32_t* final_keys = keys_out;
     if (!final_keys) {
         final_keys = static_cast<uint32_t*>(scratch(context, kSlotSortKeysB, elements * sizeof(uint32_t)));
@@ -230,9 +304,10 @@ int radix_sort_pairs(gsn_context* context, uint32_t* keys, uint32_t* values, int
                 histogram);
             GSN_TRY(check_launch("radix_histogram_kernel"));
         }
-        GSN_TRY(exclusive_scan_int(context, histogram, histogram, static_cast<int64_t>(bins) * tiles));
+        radix_digit_scan_kernel<<<bins, kSortThreads, 0, context->stream>>>(histogram, tiles, totals);
+        GSN_TRY(check_launch("radix_digit_scan_kernel"));
         radix_scatter_kernel<<<tiles, kSortThreads, 0, context->stream>>>(src_keys, src_values, n, shift, digit_bits, tiles,
-            histogram, dst_keys, dst_values);
+            histogram, totals, dst_keys, dst_values);
         GSN_TRY(check_launch("radix_scatter_kernel"));
         src_keys = dst_keys;
         src_values = dst_values;

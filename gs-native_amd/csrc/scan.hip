@@ -1,7 +1,7 @@
 // Device-wide exclusive prefix scan: per-block totals, then a downsweep in which every block adds
 // the totals of the blocks before it (summed directly up to 2048 blocks, else from a recursive
 // scan of the totals).
-// Used for record offsets (int[N+1]), density decisions (int4[N+1]) and radix-sort histograms.
+// Used for record offsets (int[N+1]), depth-order record counts and density decisions (int4[N+1]).
 #include "gsn_internal.h"
 
 namespace gsn {
