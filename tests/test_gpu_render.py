@@ -174,6 +174,48 @@ def test_tile_lists_rebuilt_without_queued_depth_order(gsn, context, scenes):
     np.testing.assert_array_equal(keys.numpy()[:r], first["sorted_keys"])
 
 
+def test_tile_lists_with_empty_rectangles_between_records(gsn, context):
+    """Inputs the projection never writes: 5,000 Gaussians marked visible by their record offsets
+    but with empty tile rectangles, between two runs of two-record Gaussians. The first
+    4096-record block then has more owners than positions, and the emission finds each record's
+    owner by binary search. The lists must equal a stable sort by (tile, depth) of the records in
+    depth order, which here is id order (depths increase with the id)."""
+    normal_before, empty, normal_after = 2000, 5000, 5000
+    count = normal_before + empty + normal_after
+    f = 600.0
+    cam = gsn.Camera.make(640, 480, f, f, 319.5, 239.5, np.eye(3), np.zeros(3))
+    tiles_x, tiles_y = cam.tile_count_x, cam.tile_count_y
+    index = np.arange(count)
+    x0 = (index * 7) % (tiles_x - 1)
+    y0 = (index * 13) % tiles_y
+    rects = np.stack([x0, y0, x0 + 2, y0 + 1], axis=1).astype(np.int32)
+    hollow = (index >= normal_before) & (index < normal_before + empty)
+    rects[hollow] = (5, 5, 5, 5)
+    depths = (1.0 + 1e-3 * index).astype(np.float32)
+    areas = (rects[:, 2] - rects[:, 0]) * (rects[:, 3] - rects[:, 1])
+    record_count = int(areas.sum())
+    tiles = np.concatenate([(rects[i, 1] * tiles_x + rects[i, 0] + np.arange(areas[i])) for i in range(count)])
+    ids = np.repeat(index, areas)
+    order = np.argsort(tiles, kind="stable")
+    expected_keys = (tiles[order].astype(np.uint64) << np.uint64(32)) | depths[ids[order]].view(np.uint32).astype(
+        np.uint64)
+    expected_offsets = np.searchsorted(tiles[order], np.arange(cam.tile_count + 1), side="left").astype(np.int64)
+
+    record_offsets = gsn.DeviceBuffer.from_numpy(np.arange(count + 1, dtype=np.int32))  # every Gaussian visible
+    out_ids = gsn.DeviceBuffer((record_count,), np.int32)
+    out_offsets = gsn.DeviceBuffer((cam.tile_count + 1,), np.int64)
+    out_keys = gsn.DeviceBuffer((record_count,), np.uint64)
+    device_rects = gsn.DeviceBuffer.from_numpy(rects)
+    device_depths = gsn.DeviceBuffer.from_numpy(depths)
+    gsn.call("gsn_build_tile_lists", context.handle, ctypes.byref(cam), count, device_rects.ptr, device_depths.ptr,
+             record_offsets.ptr, record_count, out_ids.ptr, out_offsets.ptr, out_keys.ptr)
+    context.synchronize()
+    assert record_count == 2 * (normal_before + normal_after)
+    np.testing.assert_array_equal(out_ids.numpy(), ids[order])
+    np.testing.assert_array_equal(out_offsets.numpy(), expected_offsets)
+    np.testing.assert_array_equal(out_keys.numpy(), expected_keys)
+
+
 @pytest.mark.parametrize("sh_degree", [0, 3])
 def test_forward_matches_fp32_oracle(gsn, context, oracle, scenes, sh_degree):
     params = scenes.config0_params(seed=11, count=10_000, sh_degree=3)
