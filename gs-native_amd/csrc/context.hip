@@ -320,7 +320,8 @@ int gsn_event_elapsed_ms(void* start, void* end, float* milliseconds) {
 
 namespace gsn {
 
-StageScope::StageScope(gsn_context* context, Stage stage) : context_(context), pair_(-1) {
+StageScope::StageScope(gsn_context* context, Stage stage, bool kernel_events)
+    : context_(context), pair_(-1), kernel_events_(kernel_events) {
     if (!context || !((context->timing_mask >> stage) & 1u)) {
         return;
     }
@@ -352,15 +353,23 @@ StageScope::StageScope(gsn_context* context, Stage stage) : context_(context), p
     }
     pair_ = context->event_used++;
     context->event_stage[pair_] = stage;
-    if (context->events[2 * pair_]) {
+    if (!kernel_events_ && context->events[2 * pair_]) {
         (void)hipEventRecord(context->events[2 * pair_], context->stream);
     }
 }
 
 StageScope::~StageScope() {
-    if (pair_ >= 0 && context_->events[2 * pair_ + 1]) {
+    if (pair_ >= 0 && !kernel_events_ && context_->events[2 * pair_ + 1]) {
         (void)hipEventRecord(context_->events[2 * pair_ + 1], context_->stream);
     }
+}
+
+hipEvent_t StageScope::start_event() const {
+    return pair_ >= 0 ? context_->events[2 * pair_] : nullptr;
+}
+
+hipEvent_t StageScope::stop_event() const {
+    return pair_ >= 0 ? context_->events[2 * pair_ + 1] : nullptr;
 }
 
 } // namespace gsn

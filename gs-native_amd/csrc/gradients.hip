@@ -9,6 +9,8 @@
 #include "projection.h"
 #include "raster_common.h"
 
+#include <hip/hip_ext.h>
+
 #include <cmath>
 
 namespace gsn {
@@ -692,20 +694,23 @@ __global__ __launch_bounds__(256) void zero_rows_kernel(float* moments, int coun
     }
 }
 
+// start / stop: the timing events of the stage, recorded by the dispatch itself (null when the
+// stage is not timed).
 template<int Degree, bool Fused, bool Project>
-void launch_gradients_degree(bool streaming, unsigned grid, hipStream_t stream, const int* group_order,
-    const float* partials, const int* record_offsets, const uint8_t* record_flags, uint8_t epoch, int count,
-    int record_count, const Camera& cam, float* params, float* gradients, float* screen, float* statistics,
-    float* absolute_statistics, float* moments, float lr, float c1, float c2, const NextProjection& next) {
+void launch_gradients_degree(bool streaming, unsigned grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop,
+    const int* group_order, const float* partials, const int* record_offsets, const uint8_t* record_flags,
+    uint8_t epoch, int count, int record_count, const Camera& cam, float* params, float* gradients, float* screen,
+    float* statistics, float* absolute_statistics, float* moments, float lr, float c1, float c2,
+    const NextProjection& next) {
     if (Fused && streaming) {
-        gaussian_gradients_kernel<Degree, Fused, Fused, Project><<<grid, kGradientBlock, 0, stream>>>(group_order,
-            partials, record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen,
-            statistics, absolute_statistics, moments, lr, c1, c2, next);
+        hipExtLaunchKernelGGL(gaussian_gradients_kernel<Degree, Fused, Fused, Project>, dim3(grid), dim3(kGradientBlock),
+            0, stream, start, stop, 0, group_order, partials, record_offsets, record_flags, epoch, count, record_count,
+            cam, params, gradients, screen, statistics, absolute_statistics, moments, lr, c1, c2, next);
     }
     else {
-        gaussian_gradients_kernel<Degree, Fused, false, Project><<<grid, kGradientBlock, 0, stream>>>(group_order,
-            partials, record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen,
-            statistics, absolute_statistics, moments, lr, c1, c2, next);
+        hipExtLaunchKernelGGL(gaussian_gradients_kernel<Degree, Fused, false, Project>, dim3(grid), dim3(kGradientBlock),
+            0, stream, start, stop, 0, group_order, partials, record_offsets, record_flags, epoch, count, record_count,
+            cam, params, gradients, screen, statistics, absolute_statistics, moments, lr, c1, c2, next);
     }
 }
 
@@ -722,17 +727,19 @@ int launch_gradients(gsn_context* context, const Camera& cam, int count, int rec
     const bool streaming = Fused && rows_line_aligned(params, count);
     const NextProjection none{};
     const NextProjection& target = next ? *next : none;
-    StageScope scope(context, kStageGaussianGradients);
+    StageScope scope(context, kStageGaussianGradients, true);
+    hipEvent_t start = scope.start_event();
+    hipEvent_t stop = scope.stop_event();
 #define GSN_LAUNCH_DEGREE(D)                                                                                           \
     if (Fused && next) {                                                                                               \
-        launch_gradients_degree<D, Fused, Fused>(streaming, grid, stream, group_order, partials, record_offsets,        \
-            record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics, absolute_statistics,  \
-            moments, lr, c1, c2, target);                                                                              \
+        launch_gradients_degree<D, Fused, Fused>(streaming, grid, stream, start, stop, group_order, partials,          \
+            record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,       \
+            absolute_statistics, moments, lr, c1, c2, target);                                                         \
     }                                                                                                                  \
     else {                                                                                                             \
-        launch_gradients_degree<D, Fused, false>(streaming, grid, stream, group_order, partials, record_offsets,       \
-            record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics, absolute_statistics,  \
-            moments, lr, c1, c2, target);                                                                              \
+        launch_gradients_degree<D, Fused, false>(streaming, grid, stream, start, stop, group_order, partials,          \
+            record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,       \
+            absolute_statistics, moments, lr, c1, c2, target);                                                         \
     }
     switch (clamp_sh_degree(sh_degree)) {
     case 0:

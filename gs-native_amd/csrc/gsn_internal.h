@@ -138,15 +138,22 @@ struct gsn_context {
 
 namespace gsn {
 
-// Records an event pair around the launches in its scope when timing is enabled.
+// Records an event pair around the launches in its scope when timing is enabled. A scope made
+// with kernel_events wraps exactly one kernel and records nothing itself: the kernel is launched
+// with start_event() and stop_event() through hipExtLaunchKernel, whose dispatch records them
+// (no marker packets of their own between the stage's kernel and its neighbours). Both are null
+// when the stage is not timed.
 class StageScope {
 public:
-    StageScope(gsn_context* context, Stage stage);
+    StageScope(gsn_context* context, Stage stage, bool kernel_events = false);
     ~StageScope();
+    hipEvent_t start_event() const;
+    hipEvent_t stop_event() const;
 
 private:
     gsn_context* context_;
     int pair_;
+    bool kernel_events_;
 };
 
 } // namespace gsn

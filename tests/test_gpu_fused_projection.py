@@ -66,6 +66,44 @@ def test_fused_projection_matches_the_separate_path(gsn, scenes, degree, count):
         np.testing.assert_array_equal(a, b)
 
 
+def test_gradient_stage_timed_by_its_dispatch(gsn, scenes):
+    """The gradient pass is timed by events that its own dispatch records (hipExtLaunchKernel):
+    one launch per call, a positive duration no longer than the host's bracket of the call and
+    its synchronisation, and nothing recorded once timing is off."""
+    import time
+    context = gsn.Context()
+    count = 20_000
+    host = scenes.config0_params(seed=7, count=count, sh_degree=3)
+    cam = config0_camera(gsn, 192, 160, distance=3.0)
+    image = gsn.DeviceBuffer((160, 192, 3), np.float32)
+    image_grad = gsn.DeviceBuffer.from_numpy(np.full((160, 192, 3), 1e-3, np.float32))
+    params = gsn.DeviceBuffer.from_numpy(host)
+    moments = gsn.Moments(count)
+    statistics = gsn.DeviceBuffer.zeros((2, count), np.float32)
+    absolute = gsn.DeviceBuffer.zeros((2, count), np.float32)
+    c1, c2 = gsn.adam_corrections(1)
+    try:
+        for timed in (True, False):
+            projection = gsn.render(context, params, count, cam, (0, 0, 0), image, 3)
+            context.synchronize()
+            context.set_timing(timed, stages=["gaussian_gradients"])
+            begin = time.perf_counter()
+            following = gsn.gaussian_gradients_adam_project(context, projection, params, 3, image, image_grad,
+                                                            moments, 1.6e-4, c1, c2, statistics, absolute, cam)
+            context.synchronize()
+            bracket_ms = (time.perf_counter() - begin) * 1e3
+            ms, launches = context.stage_times()["gaussian_gradients"]
+            if timed:
+                assert launches == 1 and 0.0 < ms <= bracket_ms, (ms, launches, bracket_ms)
+            else:
+                assert launches == 0
+            gsn.render_projected(context, following, (0, 0, 0), image)  # the record count comes from the scan
+            assert following.record_count > 0
+    finally:
+        context.set_timing(False)
+        context.close()
+
+
 def test_training_with_and_without_fused_projection_is_identical(gsn):
     import gs_native_amd.trainer as trainer_module
     import gs_native_amd.workloads as workloads
