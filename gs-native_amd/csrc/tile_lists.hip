@@ -14,15 +14,31 @@
 namespace gsn {
 namespace {
 
+// Four Gaussians per thread (k * 256 apart, coalesced), loads issued before the stores.
+constexpr int kDepthKeyItems = 4;
+
 __global__ __launch_bounds__(256) void depth_keys_kernel(const float* depths, const int* record_offsets, uint32_t* keys,
     uint32_t* ids, int count) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) {
-        return;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * (256 * kDepthKeyItems) + threadIdx.x;
+    float depth[kDepthKeyItems];
+    int begin[kDepthKeyItems], end[kDepthKeyItems];
+#pragma unroll
+    for (int k = 0; k < kDepthKeyItems; ++k) {
+        const int64_t i = first + k * 256;
+        if (i < count) {
+            depth[k] = depths[i];
+            begin[k] = record_offsets[i];
+            end[k] = record_offsets[i + 1];
+        }
     }
-    bool visible = record_offsets[i + 1] > record_offsets[i];
-    keys[i] = visible ? float_to_bits(depths[i]) : 0xffffffffu;
-    ids[i] = static_cast<uint32_t>(i);
+#pragma unroll
+    for (int k = 0; k < kDepthKeyItems; ++k) {
+        const int64_t i = first + k * 256;
+        if (i < count) {
+            keys[i] = end[k] > begin[k] ? float_to_bits(depth[k]) : 0xffffffffu;
+            ids[i] = static_cast<uint32_t>(i);
+        }
+    }
 }
 
 // Record counts and tile rectangles in depth order: the one gather by Gaussian id of the tile
@@ -301,8 +317,8 @@ int enqueue_depth_order(gsn_context* context, const float* depths, const int* re
         return GSN_ERROR_OUT_OF_MEMORY;
     }
     StageScope scope(context, kStageDepthSort);
-    depth_keys_kernel<<<grid_for(count, 256), 256, 0, context->stream>>>(depths, record_offsets, depth_keys, depth_ids,
-        count);
+    depth_keys_kernel<<<grid_for(count, 256 * kDepthKeyItems), 256, 0, context->stream>>>(depths, record_offsets,
+        depth_keys, depth_ids, count);
     GSN_TRY(check_launch("depth_keys_kernel"));
     return radix_sort_pairs(context, depth_keys, depth_ids, count, 32, sorted_keys, sorted_ids);
 }
