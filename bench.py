@@ -102,6 +102,8 @@ def algorithmic_bytes(*, n: float, v: float, r: float, p: int, t: int, k: int = 
         # with the next view's projection fused in: its outputs (record count for every Gaussian;
         # mean, conic + opacity, colour, depth, rectangle for the visible ones); it reads nothing more
         "gaussian_gradients_projected": 1416 * n + 56 * v + r,
+        # the density accumulators read and written by the same pass (three 4-byte values)
+        "in_pass_accumulation": 24 * n,
         # gradients written, statistics, offsets, flags; the parameter rows are read only for the
         # Gaussians that received a partial (232 bytes each, not counted: a lower bound)
         "gaussian_gradients_unfused": 252 * n + r,
@@ -384,10 +386,15 @@ def main() -> int:
     launches = timed[dominant][1]
     avg_ms = timed[dominant][0] / launches
     projected = trainer.fuse_projection
-    dominant_bytes = window_bytes["gaussian_gradients_projected" if projected else dominant]
+    # While statistics are collected (iterations below 15000) the fused pass also updates the
+    # density accumulators.
+    accumulating = sum(1 for i in range(first_timed, last_timed + 1) if i < 15000) / k_steps if projected else 0.0
+    dominant_bytes = (window_bytes["gaussian_gradients_projected" if projected else dominant]
+                      + accumulating * window_bytes["in_pass_accumulation"])
     # The same work as the reference's separate kernels: statistics, chain rule and Adam (and, with
-    # the projection fused in, the projection kernel's reads and writes).
-    separate_bytes = unfused_equivalent_bytes(n_mean, v_mean) + (window_bytes["project"] if projected else 0.0)
+    # the projection fused in, the projection kernel's reads and writes and the accumulation's).
+    separate_bytes = (unfused_equivalent_bytes(n_mean, v_mean)
+                      + (window_bytes["project"] if projected else 0.0) + accumulating * window_bytes["accumulate"])
     achieved = dominant_bytes / (avg_ms * 1e-3) / 1e9
     density_ms, density_launches = timed["density"]
     density_steps = trainer.history[history_before:]
@@ -440,7 +447,10 @@ def main() -> int:
             "algorithmic_bytes_per_launch": dominant_bytes,
             "bytes_note": "fused floor 1412N + R (Adam rows read + written, statistics, run offsets, record flags)"
                           + (" + 4N + 56V (the next view's projection outputs; V = mean visible Gaussians)"
-                             if projected else "") + "; the partials of reached records are extra",
+                             if projected else "")
+                          + (f" + 24N x {accumulating:.2f} (density accumulators read + written, in the window's "
+                             "iterations below 15000)" if accumulating else "")
+                          + "; the partials of reached records are extra",
             "unfused_equivalent_bytes_per_launch": separate_bytes,
             "unfused_equivalent_frac": separate_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },

@@ -111,7 +111,7 @@ _PROTOTYPES = {
     "gsn_gaussian_gradients_adam": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
                                     _F, _F, _F, _P, _P],
     "gsn_gaussian_gradients_adam_project": [_P, _CAM, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
-                                            _F, _F, _F, _P, _P, _CAM, _P, _P, _P, _P, _P, _P],
+                                            _F, _F, _F, _P, _P, _CAM, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "gsn_adam_update": [_P, _I, _P, _P, _P, _F, _F, _F],
     "gsn_remap_history": [_P, _P, _I, _I, _P, _P],
     "gsn_zero_rows": [_P, _P, _I, _I, _I],
@@ -581,10 +581,13 @@ def gaussian_gradients_adam_project(context: Context, projection: Projection, pa
                                     image: DeviceBuffer, image_grad: DeviceBuffer, moments: Moments,
                                     position_lr: float, first_correction: float, second_correction: float,
                                     statistics: DeviceBuffer, absolute_statistics: DeviceBuffer, next_camera: Camera,
-                                    next_projection: Projection | None = None) -> Projection:
+                                    next_projection: Projection | None = None,
+                                    accumulate: tuple | None = None) -> Projection:
     """gaussian_gradients_adam, then the projection of the updated parameters for next_camera,
     fused (see gsn_gaussian_gradients_adam_project). Returns next_projection (allocated when None)
-    with its per-Gaussian outputs written; finish the render with render_projected."""
+    with its per-Gaussian outputs written; finish the render with render_projected.
+    accumulate: None, or (gradient_sums, absolute_gradient_sums, observation_counts), to which the
+    pass adds the statistics as gsn_density_accumulate would (then do not call it for this view)."""
     p = projection
     if next_projection is None or next_projection is projection:
         next_projection = Projection(p.count, next_camera)
@@ -597,7 +600,7 @@ def gaussian_gradients_adam_project(context: Context, projection: Projection, pa
          _p(p.tile_rects), _p(p.means), _p(p.conic_opacities), _p(p.colors), _p(image), _p(image_grad), _p(moments),
          position_lr, first_correction, second_correction, _p(statistics), _p(absolute_statistics),
          ctypes.byref(next_camera), _p(q.means), _p(q.conic_opacities), _p(q.colors), _p(q.depths), _p(q.tile_rects),
-         _p(q.record_offsets))
+         _p(q.record_offsets), *(_p(b) for b in (accumulate or (None, None, None))))
     return next_projection
 
 

@@ -294,20 +294,36 @@ constexpr int kChunks = 2;  // 64-record chunks per wave pass
 // whole wave (wave_sum12) instead of one by one by its own lane.
 constexpr int kWideOwner = 16;
 
+// Density-control accumulators that a pass may add its statistics to (all null when it does not):
+// the same additions as gsn_density_accumulate makes from the written statistics.
+struct StatisticSums {
+    float* sums;
+    float* absolute_sums;
+    uint32_t* counts;
+};
+
 // Projection statistics of one Gaussian (reference CollectProjectionStatisticsKernel,
 // solver.cu:468-490): pixel-space norms of the mean gradient and of its absolute counterpart, and
 // whether the Gaussian had records. No FP contraction, so that every kernel writes identical bits.
 __device__ __forceinline__ void write_statistics(const Camera& cam, const float (&in)[11], int id, int64_t n,
-    int begin, int end, float* statistics, float* absolute_statistics) {
+    int begin, int end, float* statistics, float* absolute_statistics, const StatisticSums& accumulate) {
 #pragma clang fp contract(off)
     float x = in[0] * (0.5f * static_cast<float>(cam.width));
     float y = in[1] * (0.5f * static_cast<float>(cam.height));
     float ax = in[9] * (0.5f * static_cast<float>(cam.width));
     float ay = in[10] * (0.5f * static_cast<float>(cam.height));
-    statistics[id] = sqrtf(x * x + y * y);
-    statistics[n + id] = end > begin ? 1.0f : 0.0f;
-    absolute_statistics[id] = sqrtf(ax * ax + ay * ay);
+    const float norm = sqrtf(x * x + y * y);
+    const float visible = end > begin ? 1.0f : 0.0f;
+    const float absolute = sqrtf(ax * ax + ay * ay);
+    statistics[id] = norm;
+    statistics[n + id] = visible;
+    absolute_statistics[id] = absolute;
     absolute_statistics[n + id] = in[10];
+    if (accumulate.sums) {
+        accumulate.sums[id] += norm;
+        accumulate.absolute_sums[id] += absolute;
+        accumulate.counts[id] += static_cast<uint32_t>(visible);
+    }
 }
 
 // Sums of the stamped partials of Gaussian `id` (Gaussians at or past `limit` own no records);
@@ -482,7 +498,7 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     const int* __restrict__ record_offsets, const uint8_t* __restrict__ record_flags, uint8_t epoch, int count,
     int record_count, Camera cam, float* params, float* gradients, float* screen, float* statistics,
     float* absolute_statistics, float* moments, float position_learning_rate, float first_correction,
-    float second_correction, NextProjection next) {
+    float second_correction, NextProjection next, StatisticSums accumulate) {
     static_assert(Fused || !Project, "the next view is projected from the Adam update");
     __shared__ float4 s_records[kGradientBlock / 64][kChunks * 64 * 3];
     const int wave = threadIdx.x >> 6;
@@ -500,7 +516,7 @@ __global__ __launch_bounds__(kGradientBlock) void gaussian_gradients_kernel(cons
     if (id >= count) {
         return;
     }
-    write_statistics(cam, in, id, n, begin, end, statistics, absolute_statistics);
+    write_statistics(cam, in, id, n, begin, end, statistics, absolute_statistics, accumulate);
     if (screen) {
 #pragma unroll
         for (int k = 0; k < 11; ++k) {
@@ -701,25 +717,27 @@ void launch_gradients_degree(bool streaming, unsigned grid, hipStream_t stream, 
     const int* group_order, const float* partials, const int* record_offsets, const uint8_t* record_flags,
     uint8_t epoch, int count, int record_count, const Camera& cam, float* params, float* gradients, float* screen,
     float* statistics, float* absolute_statistics, float* moments, float lr, float c1, float c2,
-    const NextProjection& next) {
+    const NextProjection& next, const StatisticSums& accumulate) {
     if (Fused && streaming) {
         hipExtLaunchKernelGGL(gaussian_gradients_kernel<Degree, Fused, Fused, Project>, dim3(grid), dim3(kGradientBlock),
             0, stream, start, stop, 0, group_order, partials, record_offsets, record_flags, epoch, count, record_count,
-            cam, params, gradients, screen, statistics, absolute_statistics, moments, lr, c1, c2, next);
+            cam, params, gradients, screen, statistics, absolute_statistics, moments, lr, c1, c2, next, accumulate);
     }
     else {
         hipExtLaunchKernelGGL(gaussian_gradients_kernel<Degree, Fused, false, Project>, dim3(grid), dim3(kGradientBlock),
             0, stream, start, stop, 0, group_order, partials, record_offsets, record_flags, epoch, count, record_count,
-            cam, params, gradients, screen, statistics, absolute_statistics, moments, lr, c1, c2, next);
+            cam, params, gradients, screen, statistics, absolute_statistics, moments, lr, c1, c2, next, accumulate);
     }
 }
 
 // next: with Fused, the next view's projection to write from the updated parameters (or null).
+// accumulate: density-control accumulators to add the statistics to (or null).
 template<bool Fused>
 int launch_gradients(gsn_context* context, const Camera& cam, int count, int record_count, int sh_degree,
     const int* group_order, const float* partials, const int* record_offsets, const uint8_t* record_flags,
     uint8_t epoch, float* params, float* gradients, float* screen, float* statistics, float* absolute_statistics,
-    float* moments, float lr, float c1, float c2, const NextProjection* next = nullptr) {
+    float* moments, float lr, float c1, float c2, const NextProjection* next = nullptr,
+    const StatisticSums* accumulate = nullptr) {
     unsigned grid = grid_for(count, kGradientBlock);
     hipStream_t stream = context->stream;
     // Non-temporal streaming of the parameter rows only when all of them start on a 128-byte line
@@ -727,6 +745,8 @@ int launch_gradients(gsn_context* context, const Camera& cam, int count, int rec
     const bool streaming = Fused && rows_line_aligned(params, count);
     const NextProjection none{};
     const NextProjection& target = next ? *next : none;
+    const StatisticSums no_sums{};
+    const StatisticSums& sums = accumulate ? *accumulate : no_sums;
     StageScope scope(context, kStageGaussianGradients, true);
     hipEvent_t start = scope.start_event();
     hipEvent_t stop = scope.stop_event();
@@ -734,12 +754,12 @@ int launch_gradients(gsn_context* context, const Camera& cam, int count, int rec
     if (Fused && next) {                                                                                               \
         launch_gradients_degree<D, Fused, Fused>(streaming, grid, stream, start, stop, group_order, partials,          \
             record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,       \
-            absolute_statistics, moments, lr, c1, c2, target);                                                         \
+            absolute_statistics, moments, lr, c1, c2, target, sums);                                                   \
     }                                                                                                                  \
     else {                                                                                                             \
         launch_gradients_degree<D, Fused, false>(streaming, grid, stream, start, stop, group_order, partials,          \
             record_offsets, record_flags, epoch, count, record_count, cam, params, gradients, screen, statistics,       \
-            absolute_statistics, moments, lr, c1, c2, target);                                                         \
+            absolute_statistics, moments, lr, c1, c2, target, sums);                                                   \
     }
     switch (clamp_sh_degree(sh_degree)) {
     case 0:
@@ -856,7 +876,8 @@ extern "C" int gsn_gaussian_gradients_adam_project(gsn_context* context, const g
     const float* colors, const float* image, const float* image_gradients, float* moments,
     float position_learning_rate, float first_correction, float second_correction, float* statistics,
     float* absolute_statistics, const gsn_camera* next_camera, float* next_means, float* next_conic_opacities,
-    float* next_colors, float* next_depths, int* next_tile_rects, int* next_record_offsets) {
+    float* next_colors, float* next_depths, int* next_tile_rects, int* next_record_offsets, float* gradient_sums,
+    float* absolute_gradient_sums, uint32_t* observation_counts) {
     using namespace gsn;
     GSN_TRY(validate_gradient_inputs(context, camera, count, params, tile_gaussian_ids, tile_offsets, record_count,
         record_offsets, tile_rects, means, conic_opacities, colors, image, image_gradients, statistics,
@@ -869,6 +890,12 @@ extern "C" int gsn_gaussian_gradients_adam_project(gsn_context* context, const g
     if (next_camera->width <= 0 || next_camera->height <= 0 || next_camera->tile_count_x <= 0 ||
         next_camera->tile_count_y <= 0) {
         return fail(GSN_ERROR_INVALID_ARGUMENT, "gsn_gaussian_gradients_adam_project: invalid next camera");
+    }
+    const int accumulators = (gradient_sums != nullptr) + (absolute_gradient_sums != nullptr) +
+                             (observation_counts != nullptr);
+    if (accumulators != 0 && accumulators != 3) {
+        return fail(GSN_ERROR_INVALID_ARGUMENT,
+            "gsn_gaussian_gradients_adam_project: give all three density accumulators or none");
     }
     Camera cam;
     memcpy(&cam, camera, sizeof(cam));
@@ -899,9 +926,10 @@ extern "C" int gsn_gaussian_gradients_adam_project(gsn_context* context, const g
     next.depths = next_depths;
     next.tile_rects = reinterpret_cast<int4*>(next_tile_rects);
     next.record_counts = next_record_offsets;
+    const StatisticSums accumulate{gradient_sums, absolute_gradient_sums, observation_counts};
     return launch_gradients<true>(context, cam, count, record_count, sh_degree, group_order, partials,
         record_offsets, flags, epoch, params, nullptr, nullptr, statistics, absolute_statistics, moments,
-        position_learning_rate, first_correction, second_correction, &next);
+        position_learning_rate, first_correction, second_correction, &next, &accumulate);
 }
 
 extern "C" int gsn_gaussian_gradients_stored(gsn_context* context, const gsn_camera* camera, int count,

@@ -95,7 +95,8 @@ bool Solver::Step(
                             reinterpret_cast<const float*>(renderer.GetColors()), image, image_gradients_, moments_,
                             position_learning_rate, first_correction, second_correction, projection_statistics_,
                             absolute_projection_statistics_, AsCamera(next_camera), next.means, next.conic_opacities,
-                            next.colors, next.depths, next.tile_rects, next.record_offsets),
+                            next.colors, next.depths, next.tile_rects, next.record_offsets, nullptr, nullptr,
+                            nullptr),
                       "gsn_gaussian_gradients_adam_project");
         if (ok) {
             iteration_ = iteration;

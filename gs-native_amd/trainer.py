@@ -77,7 +77,8 @@ class Trainer:
     gradients and runs the separate row-major Adam kernel, with bit-identical results (used to
     time Adam on its own). overlap_backward and fuse_projection (attributes, default True) run the
     backward's preparation on the context's second stream and project the next camera's view in
-    the gradient pass; both give bit-identical results. Code that changes `params` outside the
+    the gradient pass, which then also adds the density statistics to their accumulators; both
+    give bit-identical results. Code that changes `params` outside the
     trainer's own methods must call invalidate_projection() first."""
 
     def invalidate_projection(self) -> None:
@@ -210,12 +211,17 @@ class Trainer:
         image_gradients(self.context, camera, self.image, self.targets[index], self.image_grad)
         self.adam_step += 1
         c1, c2 = adam_corrections(self.adam_step)
+        collecting = iteration < 15000  # density statistics are accumulated until iteration 15000
+        accumulated = False
         if self.fused and self.fuse_projection and min(3, (iteration + 1) // 1000) == self.sh_degree:
             next_index = self.peek_camera()
+            # The pass also adds this view's statistics to the density accumulators.
+            accumulate = (self.sums, self.absolute_sums, self.observations) if collecting else None
             self.spare_projection = gaussian_gradients_adam_project(
                 self.context, self.projection, self.params, self.sh_degree, self.image, self.image_grad,
                 self.moments, lr, c1, c2, self.statistics, self.absolute_statistics, self.cameras[next_index],
-                self.spare_projection)
+                self.spare_projection, accumulate)
+            accumulated = accumulate is not None
             self.next_view = (next_index, self.count, self.sh_degree)
         elif self.fused:
             gaussian_gradients_adam(self.context, self.projection, self.params, self.sh_degree, self.image,
@@ -230,9 +236,10 @@ class Trainer:
                  self.moments.ptr, lr, c1, c2)
         self.completed = iteration
         records = self.projection.record_count
-        if self.completed < 15000:
-            call("gsn_density_accumulate", self.context.handle, self.statistics.ptr, self.absolute_statistics.ptr,
-                 self.count, self.sums.ptr, self.absolute_sums.ptr, self.observations.ptr)
+        if collecting:
+            if not accumulated:
+                call("gsn_density_accumulate", self.context.handle, self.statistics.ptr, self.absolute_statistics.ptr,
+                     self.count, self.sums.ptr, self.absolute_sums.ptr, self.observations.ptr)
             if self.features_enabled:
                 call("gsn_density_features_accumulate", self.context.handle, self.statistics.ptr,
                      self.absolute_statistics.ptr, self.projection.depths.ptr, self.projection.conic_opacities.ptr,

@@ -258,7 +258,12 @@ int gsn_gaussian_gradients_adam(gsn_context* context, const gsn_camera* camera, 
  * next_record_offsets must not be record_offsets; the other next_* buffers may be the current
  * projection's (the gradient pass does not read them). Replaces Solver::Iterate
  * (solver.cu:702-720) and the projection of the following Renderer::Render (renderer.cu:374-444)
- * when nothing changes the parameters in between (density control and opacity caps do). */
+ * when nothing changes the parameters in between (density control and opacity caps do).
+ * gradient_sums, absolute_gradient_sums, observation_counts: all null, or the density
+ * accumulators, to which the pass adds this view's statistics exactly as a following
+ * gsn_density_accumulate would (the statistics are written either way); a caller that passes
+ * them does not call gsn_density_accumulate for this view (replaces
+ * AccumulateRoundStatisticsKernel, density_controller.cu:280-295, for this view). */
 int gsn_gaussian_gradients_adam_project(gsn_context* context, const gsn_camera* camera, int count,
     int sh_degree, float* params, const int* tile_gaussian_ids, const int64_t* tile_offsets,
     int record_count, const int* record_offsets, const int* tile_rects, const float* means,
@@ -266,7 +271,8 @@ int gsn_gaussian_gradients_adam_project(gsn_context* context, const gsn_camera* 
     const float* image_gradients, float* moments, float position_learning_rate,
     float first_correction, float second_correction, float* statistics, float* absolute_statistics,
     const gsn_camera* next_camera, float* next_means, float* next_conic_opacities, float* next_colors,
-    float* next_depths, int* next_tile_rects, int* next_record_offsets);
+    float* next_depths, int* next_tile_rects, int* next_record_offsets, float* gradient_sums,
+    float* absolute_gradient_sums, uint32_t* observation_counts);
 
 /* Dense Adam over all 58*N entries with the reference's per-group learning rates.
  * Replaces UpdateGaussiansKernel (solver.cu:651-689, 901-929). */

@@ -1,7 +1,8 @@
 """The fused gradient pass can also project the updated parameters for the next view
 (gsn_gaussian_gradients_adam_project). These tests require it to write what the separate path
-(gsn_gaussian_gradients_adam, then gsn_project_gaussians of the updated parameters) writes, bit
-for bit: parameters, moments, every projection output of the visible Gaussians, and the tile
+(gsn_gaussian_gradients_adam and gsn_density_accumulate, then gsn_project_gaussians of the updated
+parameters) writes, bit for bit: parameters, moments, density accumulators, every projection
+output of the visible Gaussians, and the tile
 lists and image rendered from them; for SH degrees 0-3, with parameter rows on and off the
 128-byte line grid. A training sequence across iteration 1000 (SH degree change, density step,
 opacity cap, each of which drops the projected view) must match the trainer without fusion.
@@ -31,27 +32,39 @@ def test_fused_projection_matches_the_separate_path(gsn, scenes, degree, count):
     rng = np.random.default_rng(degree)
     image_grad = gsn.DeviceBuffer.from_numpy(rng.normal(0, 1e-3, (160, 192, 3)).astype(np.float32))
     c1, c2 = gsn.adam_corrections(3)
+    # Density accumulators part-way through a run: the fused pass adds this view's statistics as
+    # gsn_density_accumulate does after the separate pass.
+    start_sums = rng.uniform(0.0, 1e-2, (2, count)).astype(np.float32)
+    start_counts = rng.integers(0, 50, count).astype(np.uint32)
     results = []
     for fused in (False, True):
         params = gsn.DeviceBuffer.from_numpy(host)
         moments = gsn.Moments(count)
         statistics = gsn.DeviceBuffer.zeros((2, count), np.float32)
         absolute = gsn.DeviceBuffer.zeros((2, count), np.float32)
+        sums = gsn.DeviceBuffer.from_numpy(start_sums[0])
+        absolute_sums = gsn.DeviceBuffer.from_numpy(start_sums[1])
+        observations = gsn.DeviceBuffer.from_numpy(start_counts)
         projection = gsn.render(context, params, count, cam, (0, 0, 0), image, degree)
         out_image = gsn.DeviceBuffer((160, 192, 3), np.float32)
         if fused:
             following = gsn.gaussian_gradients_adam_project(context, projection, params, degree, image, image_grad,
-                                                            moments, 1.6e-4, c1, c2, statistics, absolute, next_cam)
+                                                            moments, 1.6e-4, c1, c2, statistics, absolute, next_cam,
+                                                            accumulate=(sums, absolute_sums, observations))
             gsn.render_projected(context, following, (0, 0, 0), out_image)
         else:
             gsn.gaussian_gradients_adam(context, projection, params, degree, image, image_grad, moments, 1.6e-4, c1,
                                         c2, statistics, absolute)
+            gsn.call("gsn_density_accumulate", context.handle, statistics.ptr, absolute.ptr, count, sums.ptr,
+                     absolute_sums.ptr, observations.ptr)
             following = gsn.render(context, params, count, next_cam, (0, 0, 0), out_image, degree)
         context.synchronize()
         counts = np.diff(following.record_offsets.numpy())
         results.append({
             "params": params.numpy().view(np.uint32), "moments": [m.view(np.uint32) for m in moments.numpy()],
             "statistics": statistics.numpy().view(np.uint32), "offsets": following.record_offsets.numpy(),
+            "sums": np.stack([sums.numpy(), absolute_sums.numpy()]).view(np.uint32),
+            "observations": observations.numpy(),
             "counts": counts, "records": following.record_count,
             "ids": following.tile_gaussian_ids.numpy()[:following.record_count],
             "tile_offsets": following.tile_offsets.numpy(), "image": out_image.numpy().view(np.uint32),
@@ -59,8 +72,9 @@ def test_fused_projection_matches_the_separate_path(gsn, scenes, degree, count):
                for name in ("means", "conic_opacities", "colors", "depths", "tile_rects")}})
     separate, fused = results
     assert separate["records"] > 0 and (separate["counts"] > 0).mean() > 0.5
-    for key in ("params", "statistics", "offsets", "ids", "tile_offsets", "image", "means", "conic_opacities",
-                "colors", "depths", "tile_rects"):
+    assert (separate["observations"] > start_counts).any()
+    for key in ("params", "statistics", "sums", "observations", "offsets", "ids", "tile_offsets", "image", "means",
+                "conic_opacities", "colors", "depths", "tile_rects"):
         np.testing.assert_array_equal(fused[key], separate[key], err_msg=key)
     for a, b in zip(fused["moments"], separate["moments"]):
         np.testing.assert_array_equal(a, b)
