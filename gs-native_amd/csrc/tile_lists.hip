@@ -14,30 +14,46 @@
 namespace gsn {
 namespace {
 
-// Four Gaussians per thread (k * 256 apart, coalesced), loads issued before the stores.
-constexpr int kDepthKeyItems = 4;
+// One 4096-key radix tile per workgroup (512 threads, eight Gaussians each, k * 512 apart so each
+// access stays coalesced; loads issued before the stores). The workgroup also counts its keys'
+// low eight bits, the first depth-sort pass's histogram column, so that pass needs no histogram
+// kernel.
+constexpr int kDepthKeyThreads = 512;
+constexpr int kDepthKeyItems = kRadixTile / kDepthKeyThreads;
 
-__global__ __launch_bounds__(256) void depth_keys_kernel(const float* depths, const int* record_offsets, uint32_t* keys,
-    uint32_t* ids, int count) {
-    const int64_t first = static_cast<int64_t>(blockIdx.x) * (256 * kDepthKeyItems) + threadIdx.x;
+__global__ __launch_bounds__(kDepthKeyThreads) void depth_keys_kernel(const float* depths, const int* record_offsets,
+    uint32_t* keys, uint32_t* ids, int count, int tiles, int* histogram) {
+    __shared__ int s_digits[256];
+    const int t = threadIdx.x;
+    if (t < 256) {
+        s_digits[t] = 0;
+    }
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kRadixTile + t;
     float depth[kDepthKeyItems];
     int begin[kDepthKeyItems], end[kDepthKeyItems];
 #pragma unroll
     for (int k = 0; k < kDepthKeyItems; ++k) {
-        const int64_t i = first + k * 256;
+        const int64_t i = first + k * kDepthKeyThreads;
         if (i < count) {
             depth[k] = depths[i];
             begin[k] = record_offsets[i];
             end[k] = record_offsets[i + 1];
         }
     }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kDepthKeyItems; ++k) {
-        const int64_t i = first + k * 256;
+        const int64_t i = first + k * kDepthKeyThreads;
         if (i < count) {
-            keys[i] = end[k] > begin[k] ? float_to_bits(depth[k]) : 0xffffffffu;
+            const uint32_t key = end[k] > begin[k] ? float_to_bits(depth[k]) : 0xffffffffu;
+            keys[i] = key;
             ids[i] = static_cast<uint32_t>(i);
+            atomicAdd(&s_digits[key & 0xffu], 1);
         }
+    }
+    __syncthreads();
+    if (t < 256) {
+        histogram[static_cast<int64_t>(t) * tiles + blockIdx.x] = s_digits[t];
     }
 }
 
@@ -316,11 +332,18 @@ int enqueue_depth_order(gsn_context* context, const float* depths, const int* re
     if (!depth_keys || !depth_ids || !sorted_ids || !sorted_keys) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
+    int* histogram = radix_histogram_buffer(context, count);
+    if (!histogram) {
+        return GSN_ERROR_OUT_OF_MEMORY;
+    }
+    static_assert(kDepthKeyThreads * kDepthKeyItems == kRadixTile, "one radix tile per workgroup");
+    const int tiles = static_cast<int>((static_cast<int64_t>(count) + kRadixTile - 1) / kRadixTile);
     StageScope scope(context, kStageDepthSort);
-    depth_keys_kernel<<<grid_for(count, 256 * kDepthKeyItems), 256, 0, context->stream>>>(depths, record_offsets,
-        depth_keys, depth_ids, count);
+    // 32 key bits: the first pass takes radix_first_digit_bits(32) = 8 bits, the low byte counted here.
+    depth_keys_kernel<<<tiles, kDepthKeyThreads, 0, context->stream>>>(depths, record_offsets, depth_keys, depth_ids,
+        count, tiles, histogram);
     GSN_TRY(check_launch("depth_keys_kernel"));
-    return radix_sort_pairs(context, depth_keys, depth_ids, count, 32, sorted_keys, sorted_ids);
+    return radix_sort_pairs(context, depth_keys, depth_ids, count, 32, sorted_keys, sorted_ids, true);
 }
 
 } // namespace gsn
