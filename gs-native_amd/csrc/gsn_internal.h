@@ -41,6 +41,7 @@ enum ScratchSlot {
     kSlotGaussianOrder,
     kSlotQuantizedError,
     kSlotKnnOrder,
+    kSlotDepthCountTotals,
     kSlotCount
 };
 
@@ -181,6 +182,12 @@ struct Int4 {
     int x, y, z, w;
 };
 int exclusive_scan_int(gsn_context* context, const int* input, int* output, int64_t n);
+// The same scan when the producer of `input` has written the sum of each kScanBlock-element block
+// (entries [b * kScanBlock, (b + 1) * kScanBlock) of [0, n)) to block_totals[b]: only the
+// downsweep runs (beyond 2048 blocks the totals are ignored and the full scan runs).
+constexpr int kScanBlock = 4096;
+int exclusive_scan_int_from_totals(gsn_context* context, const int* input, int* output, int64_t n,
+    const int* block_totals);
 int exclusive_scan_int4(gsn_context* context, const Int4* input, Int4* output, int64_t n);
 
 // Stable LSD radix sort of (uint32 key, uint32 value) pairs over key bits [0, bits)

@@ -12,6 +12,7 @@ namespace {
 constexpr int kScanThreads = 512;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
+static_assert(kScanTile == kScanBlock, "block totals written by producers cover kScanBlock entries");
 
 __device__ __forceinline__ int add(int a, int b) {
     return a + b;
@@ -252,6 +253,18 @@ int exclusive_scan(gsn_context* context, const T* input, T* output, int64_t n, i
 
 int exclusive_scan_int(gsn_context* context, const int* input, int* output, int64_t n) {
     return exclusive_scan<int>(context, input, output, n, 0);
+}
+
+int exclusive_scan_int_from_totals(gsn_context* context, const int* input, int* output, int64_t n,
+    const int* block_totals) {
+    const int64_t blocks = (n + kScanTile - 1) / kScanTile;
+    if (n <= 0 || blocks == 1 || blocks > kSumTotalsBlocks) {
+        return exclusive_scan<int>(context, input, output, n, 0);
+    }
+    const int vector = (reinterpret_cast<uintptr_t>(input) % 16 == 0 && reinterpret_cast<uintptr_t>(output) % 16 == 0) ? 1 : 0;
+    scan_downsweep_kernel<int><<<static_cast<unsigned>(blocks), kScanThreads, 0, context->stream>>>(input, output,
+        block_totals, n, 1, vector);
+    return check_launch("scan_downsweep_kernel");
 }
 
 int exclusive_scan_int4(gsn_context* context, const Int4* input, Int4* output, int64_t n) {

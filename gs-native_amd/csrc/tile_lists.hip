@@ -61,22 +61,52 @@ __global__ __launch_bounds__(kDepthKeyThreads) void depth_keys_kernel(const floa
 // lists. A visible Gaussian's record count is its rectangle's area; invisible Gaussians (sorted
 // last, with the key 0xffffffff that no finite depth has) get none, and their rectangles, which
 // the projection leaves unwritten, are not read.
-__global__ __launch_bounds__(256) void depth_order_counts_kernel(const uint32_t* sorted_ids, const uint32_t* sorted_keys,
-    const int4* tile_rects, int* counts, int4* ordered_rects, int count) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > count) {
-        return;
+// One kScanBlock-entry block of [0, count] per workgroup (512 threads, eight entries each, k * 512
+// apart): the workgroup also sums its counts into block_totals[block], the first half of the
+// counts' exclusive scan.
+constexpr int kOrderThreads = 512;
+constexpr int kOrderItems = kScanBlock / kOrderThreads;
+
+__global__ __launch_bounds__(kOrderThreads) void depth_order_counts_kernel(const uint32_t* sorted_ids,
+    const uint32_t* sorted_keys, const int4* tile_rects, int* counts, int4* ordered_rects, int count,
+    int* block_totals) {
+    __shared__ int s_wave_sums[kOrderThreads / 64];
+    const int t = threadIdx.x;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kScanBlock + t;
+    int sum = 0;
+#pragma unroll
+    for (int j = 0; j < kOrderItems; ++j) {
+        const int64_t k = first + j * kOrderThreads;
+        if (k < count) {
+            int4 rect = make_int4(0, 0, 0, 0);
+            if (sorted_keys[k] != 0xffffffffu) {
+                rect = tile_rects[min(sorted_ids[k], static_cast<uint32_t>(count - 1))];
+            }
+            ordered_rects[k] = rect;
+            const int records = max(rect.z - rect.x, 0) * max(rect.w - rect.y, 0);
+            counts[k] = records;
+            sum += records;
+        }
+        else if (k == count) {
+            counts[k] = 0;
+        }
     }
-    if (k == count) {
-        counts[k] = 0;
-        return;
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        sum += __shfl_xor(sum, offset, 64);
     }
-    int4 rect = make_int4(0, 0, 0, 0);
-    if (sorted_keys[k] != 0xffffffffu) {
-        rect = tile_rects[min(sorted_ids[k], static_cast<uint32_t>(count - 1))];
+    if ((t & 63) == 0) {
+        s_wave_sums[t >> 6] = sum;
     }
-    ordered_rects[k] = rect;
-    counts[k] = max(rect.z - rect.x, 0) * max(rect.w - rect.y, 0);
+    __syncthreads();
+    if (t == 0) {
+        int total = 0;
+#pragma unroll
+        for (int w = 0; w < kOrderThreads / 64; ++w) {
+            total += s_wave_sums[w];
+        }
+        block_totals[blockIdx.x] = total;
+    }
 }
 
 // Records are emitted in blocks of kRadixTile consecutive positions, one workgroup per block, so
@@ -432,15 +462,18 @@ extern "C" int gsn_build_tile_lists(gsn_context* context, const gsn_camera* came
     uint32_t* sorted_tiles = static_cast<uint32_t*>(scratch(context, kSlotSortKeysB, r * 4));
     int* block_owners = static_cast<int*>(scratch(context, kSlotEmitOwners, (static_cast<size_t>(blocks) + 1) * 4));
     int* histogram = radix_histogram_buffer(context, record_count);
-    if (!sorted_tiles || !block_owners || !histogram) {
+    const int count_blocks = static_cast<int>((static_cast<int64_t>(count) + kScanBlock) / kScanBlock);  // count + 1 entries
+    int* count_totals = static_cast<int*>(scratch(context, kSlotDepthCountTotals, static_cast<size_t>(count_blocks) * 4));
+    if (!sorted_tiles || !block_owners || !histogram || !count_totals) {
         return GSN_ERROR_OUT_OF_MEMORY;
     }
     {
         StageScope scope(context, kStageEmit);
-        depth_order_counts_kernel<<<grid_for(count + 1, 256), 256, 0, stream>>>(sorted_ids, depth_sorted_keys,
-            reinterpret_cast<const int4*>(tile_rects), depth_counts, ordered_rects, count);
+        depth_order_counts_kernel<<<count_blocks, kOrderThreads, 0, stream>>>(sorted_ids, depth_sorted_keys,
+            reinterpret_cast<const int4*>(tile_rects), depth_counts, ordered_rects, count, count_totals);
         GSN_TRY(check_launch("depth_order_counts_kernel"));
-        GSN_TRY(exclusive_scan_int(context, depth_counts, depth_counts, static_cast<int64_t>(count) + 1));
+        GSN_TRY(exclusive_scan_int_from_totals(context, depth_counts, depth_counts, static_cast<int64_t>(count) + 1,
+            count_totals));
         emit_owners_kernel<<<grid_for(count, 256), 256, 0, stream>>>(depth_counts, count, record_count, blocks,
             block_owners);
         GSN_TRY(check_launch("emit_owners_kernel"));
